@@ -1,0 +1,218 @@
+"""Workload definitions: grids, boundary conditions and initial fields.
+
+Each builder configures an ``ibm3d.Solver`` through the reference's C API
+exactly like the reference drivers do (src/main.c:133-159, src/cavity.c:50-
+77): grid, params, BCs, obstacle, linear solver, assemble, init.
+
+Workloads:
+  cavity   3-D lid-driven cavity on the unit cube, lid at y = 1 (DIR_NORTH)
+           moving with u = 1, stationary walls elsewhere; u = v = w = p = 0
+           plus a seeded uniform perturbation of amplitude 1e-3 inside.
+  sphere   flow past a sphere of diameter 1 at the origin, phi = |x| - 0.5,
+           domain [-5,15]x[-5,5]^2, cosine-stretched toward the body; West
+           velocity inlet (1,0,0), East pressure outlet p = 0, free-slip
+           lateral walls; initial u = (1,0,0), p = 0 (src/main.c:139-155).
+"""
+from __future__ import annotations
+
+import math
+import struct
+
+import numpy as np
+
+from . import ibm3d as I
+
+MASK = (1 << 64) - 1
+
+
+# ---------------------------------------------------------------------------
+# seeded perturbation: counter-based SplitMix64 over the bit patterns of the
+# cell-centre coordinates, so the draw is identical on every backend and does
+# not depend on traversal order.
+# ---------------------------------------------------------------------------
+def _sm64(x: int) -> int:
+    x = (x + 0x9E3779B97F4A7C15) & MASK
+    z = x
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+    return z ^ (z >> 31)
+
+
+def _bits(d: float) -> int:
+    return struct.unpack("<Q", struct.pack("<d", d))[0]
+
+
+def uniform01(seed: int, comp: int, x: float, y: float, z: float) -> float:
+    h = _sm64(_bits(z))
+    h = _sm64(_bits(y) ^ h)
+    h = _sm64(_bits(x) ^ h)
+    h = _sm64(comp ^ h)
+    h = _sm64(seed ^ h)
+    return (h >> 11) * (1.0 / (1 << 53))
+
+
+def _sm64_np(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        z = x
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01_grid(seed: int, comp: int, xc, yc, zc) -> np.ndarray:
+    """Vectorised uniform01 over the tensor grid xc x yc x zc (k fastest)."""
+    bx = np.asarray(xc, dtype=np.float64).view(np.uint64)[:, None, None]
+    by = np.asarray(yc, dtype=np.float64).view(np.uint64)[None, :, None]
+    bz = np.asarray(zc, dtype=np.float64).view(np.uint64)[None, None, :]
+    h = _sm64_np(np.broadcast_to(bz, (len(xc), len(yc), len(zc))).copy())
+    h = _sm64_np(by ^ h)
+    h = _sm64_np(bx ^ h)
+    h = _sm64_np(np.uint64(comp) ^ h)
+    h = _sm64_np(np.uint64(seed) ^ h)
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+# ---------------------------------------------------------------------------
+# grids
+# ---------------------------------------------------------------------------
+def uniform_faces(a: float, b: float, n: int) -> np.ndarray:
+    return np.array([a + (b - a) * i / n for i in range(n + 1)])
+
+
+def cosspace(a: float, b: float, n: int) -> np.ndarray:
+    """Cosine clustering toward both ends (the reference's cosspace.c:14-17)."""
+    return np.array([(b - a) * (1 - math.cos(math.pi * i / n)) / 2 + a for i in range(n + 1)])
+
+
+def _partial_cos(L: float, n: int, theta0: float) -> np.ndarray:
+    """n cells on [0, L] clustered toward 0 by a cosine over [theta0, pi/2]:
+    x(s) = L (cos t0 - cos(t0 + s (pi/2 - t0))) / cos t0."""
+    s = np.arange(n + 1) / n
+    th = theta0 + s * (math.pi / 2 - theta0)
+    x = L * (math.cos(theta0) - np.cos(th)) / math.cos(theta0)
+    x[0], x[-1] = 0.0, L
+    return x
+
+
+def toward_point_faces(a: float, b: float, n: int, c: float, dmin: float) -> np.ndarray:
+    """Faces on [a, b] clustered toward the interior point c with smallest cell
+    width ~dmin on both sides: a partial-angle cosine map per side, the cell
+    count split so that both sides start from the same width. This is the
+    pinned stretching of the sphere workload (the reference's cosspace.c only
+    clusters toward the domain ends)."""
+    La, Lb = c - a, b - c
+
+    def theta_for(L, m):
+        lo, hi = 1e-6, math.pi / 2 - 1e-6
+        for _ in range(200):
+            mid = 0.5 * (lo + hi)
+            if _partial_cos(L, m, mid)[1] < dmin:
+                lo = mid
+            else:
+                hi = mid
+        return 0.5 * (lo + hi)
+
+    best = None
+    for na in range(2, n - 1):
+        nb = n - na
+        ta, tb = theta_for(La, na), theta_for(Lb, nb)
+        xa, xb = _partial_cos(La, na, ta), _partial_cos(Lb, nb, tb)
+        # match the largest widths of the two sides as well as possible
+        score = abs(math.log((xa[-1] - xa[-2]) / (xb[-1] - xb[-2])))
+        if best is None or score < best[0]:
+            best = (score, xa, xb)
+    _, xa, xb = best
+    faces = np.concatenate([c - xa[::-1], c + xb[1:]])
+    faces[0], faces[-1] = a, b
+    return faces
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+CAVITY_SEED = 2020
+
+
+def setup_cavity(s: I.Solver, n: int = 32, Re: float = 100.0, dt: float = 0.005,
+                 tol: float = 1e-10, seed: int = CAVITY_SEED, amp: float = 1e-3,
+                 precision: int = I.PREC_FP64, init: str = "array"):
+    f = uniform_faces(0.0, 1.0, n)
+    s.set_grid(f, f, f)
+    s.set_params(Re, dt)
+    s.set_bc(I.DIR_NORTH, I.BC_VELOCITY_COMPONENT, I.VAL_CONST, 1.0, 0.0, 0.0)
+    s.set_bc(I.DIR_EAST | I.DIR_WEST | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP, I.BC_STATIONARY_WALL)
+    s.set_obstacle(None)
+    s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, tol)
+    s.set_precision(precision)
+    s.assemble()
+
+    def pert(comp):
+        def f3(x, y, z):
+            if 0.0 < x < 1.0 and 0.0 < y < 1.0 and 0.0 < z < 1.0:
+                return amp * (2.0 * uniform01(seed, comp, x, y, z) - 1.0)
+            return 0.0
+        return f3
+
+    if init == "func":
+        s.init_flow_func(pert(1), pert(2), pert(3), lambda x, y, z: 0.0)
+    else:
+        s.init_flow_const(0.0, 0.0, 0.0, 0.0)
+        xc, yc, zc = cell_centres(f), cell_centres(f), cell_centres(f)
+        inside = (((xc > 0) & (xc < 1))[:, None, None] & ((yc > 0) & (yc < 1))[None, :, None]
+                  & ((zc > 0) & (zc < 1))[None, None, :])
+        for comp, name in ((1, "U1"), (2, "U2"), (3, "U3")):
+            a = amp * (2.0 * uniform01_grid(seed, comp, xc, yc, zc) - 1.0)
+            s.set(name, np.where(inside, a, 0.0))
+    return s
+
+
+def cell_centres(faces: np.ndarray, periodic: bool = False) -> np.ndarray:
+    """Padded cell centres (2 ghost layers) as IBMSolver_assemble builds them
+    for non-periodic axes (src/ibm3d_setup.c:536-588)."""
+    f = np.asarray(faces, dtype=np.float64)
+    n = len(f) - 1
+    d = np.zeros(n + 4)
+    c = np.zeros(n + 4)
+    d[2:n + 2] = f[1:] - f[:-1]
+    c[2:n + 2] = (f[1:] + f[:-1]) / 2
+    if periodic:
+        d[1], d[0], d[n + 2], d[n + 3] = d[n + 1], d[n], d[2], d[3]
+    else:
+        d[1], d[0], d[n + 2], d[n + 3] = d[2], d[3], d[n + 1], d[n]
+    c[1] = c[2] - (d[1] + d[2]) / 2
+    c[0] = c[1] - (d[0] + d[1]) / 2
+    c[n + 2] = c[n + 1] + (d[n + 1] + d[n + 2]) / 2
+    c[n + 3] = c[n + 2] + (d[n + 2] + d[n + 3]) / 2
+    return c
+
+
+SPHERE_DMIN = 0.015
+
+
+def sphere_faces(nx: int = 256, ny: int = 128, nz: int = 128, dmin: float = SPHERE_DMIN):
+    return (toward_point_faces(-5.0, 15.0, nx, 0.0, dmin),
+            toward_point_faces(-5.0, 5.0, ny, 0.0, dmin),
+            toward_point_faces(-5.0, 5.0, nz, 0.0, dmin))
+
+
+def setup_sphere(s: I.Solver, nx: int = 256, ny: int = 128, nz: int = 128, Re: float = 300.0,
+                 dt: float = 0.002, tol: float = 1e-10, precision: int = I.PREC_FP64,
+                 dmin: float | None = None, faces=None):
+    if faces is None:
+        if dmin is None:
+            # keep the cell-width ratio of the 256x128x128 grid on coarser grids
+            dmin = SPHERE_DMIN * 128.0 / ny
+        faces = sphere_faces(nx, ny, nz, dmin)
+    xf, yf, zf = faces
+    s.set_grid(xf, yf, zf)
+    s.set_params(Re, dt)
+    s.set_bc(I.DIR_WEST, I.BC_VELOCITY_COMPONENT, I.VAL_CONST, 1.0, 0.0, 0.0)
+    s.set_bc(I.DIR_EAST, I.BC_PRESSURE, I.VAL_CONST, 0.0)
+    s.set_bc(I.DIR_NORTH | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP, I.BC_FREE_SLIP_WALL)
+    s.set_obstacle_sphere(0.0, 0.0, 0.0, 0.5)
+    s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, tol)
+    s.set_precision(precision)
+    s.assemble()
+    s.init_flow_const(1.0, 0.0, 0.0, 0.0)
+    return s

@@ -1,0 +1,210 @@
+// solver.h - host-side state of one IBMSolver instance (opaque to callers).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/ibm3d.h"
+#include "common.h"
+#include "solver_kernels.h"
+#include "step_kernels.h"
+
+namespace ibm {
+
+// Device buffer owned by the solver.
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) { free(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { free(); }
+    void alloc(size_t cnt) {
+        free();
+        n = cnt;
+        if (cnt) {
+            HIP_CHECK(hipMalloc(&p, cnt * sizeof(T)));
+            HIP_CHECK(hipMemset(p, 0, cnt * sizeof(T)));
+        }
+    }
+    void upload(const T *h, size_t cnt) {
+        if (cnt > n) alloc(cnt);
+        if (cnt) HIP_CHECK(hipMemcpy(p, h, cnt * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void upload(const std::vector<T> &h) { upload(h.data(), h.size()); }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct HostBC {
+    IBMSolverBCType type = BC_STATIONARY_WALL;
+    IBMSolverValType val_type = VAL_CONST;
+    double cu[3] = {0, 0, 0}, cp = 0;
+    IBMSolverBCValFunc fu[3] = {nullptr, nullptr, nullptr}, fp = nullptr;
+    bool set = false;
+};
+
+// Per-axis folded coefficients of one system, both precisions.
+struct SysData {
+    DBuf<double> c64[3][4];   // cm, cp, dm, dp per axis
+    DBuf<float> c32[3][4];
+    SysCoef<double> S64{};
+    SysCoef<float> S32{};
+    double gamma[3][2] = {};  // RHS fold factor per face
+    double bc_norm2 = 0;      // squared norm of the BC rows' RHS (constant data)
+};
+
+struct GhostData {
+    DBuf<long> cell, src;
+    DBuf<double> w64, ws64;
+    DBuf<float> w32, ws32;
+    GhostRows<double> G64{};
+    GhostRows<float> G32{};
+};
+
+// One multigrid level of the pressure system.
+struct MGLevel {
+    Dims d{};
+    int cx = 0, cy = 0, cz = 0;       // axes coarsened relative to the finer level
+    std::vector<double> dx, dy, dz, xc, yc, zc;  // padded (+2)
+    std::vector<uint8_t> hflag;
+    DBuf<uint8_t> flag;
+    DBuf<double> dxyz;                 // dx | dy | dz padded
+    SysData sys;
+    DBuf<double> x64, b64, r64;
+    DBuf<float> x32, b32, r32;
+    // transfer to this level from the finer one (indices over the finer grid)
+    DBuf<int> pi, pj, pk;
+    DBuf<double> wi, wj, wk;
+    MGTransfer tr{};
+};
+
+struct KTimer {
+    std::string name;
+    double bytes = 0;       // algorithmic bytes per launch
+    double total_ms = 0;
+    long count = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct Krylov {
+    DBuf<double> v64[8];
+    DBuf<float> v32[8];
+};
+
+}  // namespace ibm
+
+struct _ibm_solver {
+    // ---- configuration (reference fields, src/ibm3d.h:100-213) ----
+    int Nx = 0, Ny = 0, Nz = 0;
+    double Re = 1, dt = 1;
+    int iter = 0;
+    double time = 0;
+    const char *autosave_filename = nullptr;
+    int autosave_period = 0;
+    std::vector<double> xf, yf, zf;
+    std::vector<double> dx, dy, dz, xc, yc, zc;           // padded (+2)
+    std::vector<double> kW, kE, kS, kN, kD, kU;           // padded (+2)
+    double xmin = 0, xmax = 0, ymin = 0, ymax = 0, zmin = 0, zmax = 0;
+    ibm::HostBC bc[6];
+    IBMSolverValType force_type = VAL_CONST;
+    double const_f[3] = {0, 0, 0};
+    IBMSolverForceFunc func_f[3] = {nullptr, nullptr, nullptr};
+    IBMSolverTurbModelType turb_type = TURBMODEL_NONE;
+    double turb_coef = 0;
+    Polyhedron *poly = nullptr;
+    int has_sphere = 0;
+    double sphere[4] = {0, 0, 0, 0};
+    IBMSolverLinearSolverType solver_type = SOLVER_BiCGSTAB;
+    IBMSolverPrecondType precond_type = PRECOND_AMG;
+    double tol = 1e-6;
+    int precision = IBM3D_PREC_FP64;
+    bool assembled = false;
+    bool p_singular = true;
+
+    // ---- host mirrors ----
+    std::vector<double> lvset_h;     // reference layout (k fastest)
+    std::vector<int> flag_ref;       // reference layout
+    std::vector<uint8_t> flag_dev_h; // device layout
+
+    // ---- device state ----
+    hipStream_t stream = nullptr;
+    ibm::Dims d{};
+    ibm::Geo geo{};
+    ibm::DBuf<double> g1d[12];       // dx dy dz xc yc zc kW kE kS kN kD kU
+    ibm::DBuf<double> ks1d[3];       // kW+kE, kS+kN, kD+kU (interior, unpadded)
+    ibm::DBuf<uint8_t> flag;
+    ibm::DBuf<double> lvset;
+    ibm::DBuf<double> u[3], un[3], us[3], F[3], Fn[3], Fs[3], N[3], Np[3], b[3];
+    ibm::DBuf<double> p, pn, pp, bp;
+    ibm::DBuf<double> tau[6];
+    ibm::DBuf<double> force_arr[3];
+    ibm::DBuf<double> bc_arr[3][2];  // host-evaluated BC data of VAL_FUNC faces
+    ibm::DBuf<double> partial, sums;
+    ibm::DBuf<ibm::KScal> ks;
+    ibm::KScal *ks_h = nullptr;      // pinned
+    double *sums_h = nullptr;        // pinned
+
+    // ghost cells
+    int ng = 0;
+    std::vector<ibm::GhostCell> ghosts;   // level-sorted
+    std::vector<int> ghost_lex;           // lexicographic order -> slot
+    std::vector<int> level_start;
+    ibm::DBuf<ibm::GhostCell> gc;
+    ibm::DBuf<double> ghost_old;
+    ibm::GhostData gmom, gp;              // solver ghost rows (momentum / pressure)
+
+    ibm::SysData sys[4];                  // u1, u2, u3, p
+    long pin = -1;
+    std::vector<ibm::MGLevel> mg;         // mg[0] = fine pressure level
+    ibm::Krylov kry;
+    ibm::DBuf<double> ref_r64;            // mixed mode: FP64 defect
+    ibm::DBuf<float> ref_rhs32, ref_e32;  // mixed mode: FP32 defect and correction
+
+    // diagnostics
+    double final_norm[4] = {0, 0, 0, 0};
+    int final_iter[4] = {0, 0, 0, 0};
+    bool timing = false;
+    std::vector<ibm::KTimer> timers;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace ibm {
+
+// api.cpp
+long host_ref_index(const IBMSolver *s, int i, int j, int k);
+void fatal(const char *msg);
+void build_system(const IBMSolver *s, const std::vector<double> *D[3], const std::vector<double> *C[3],
+                  const std::vector<double> *Km[3], const std::vector<double> *Kp[3], const int n[3],
+                  const IBMSolverBCType bct[3][2], int sk, SysData &out);
+
+// step.cpp
+void iterate_impl(IBMSolver *s, int steps, bool verbose);
+void build_mg(IBMSolver *s);
+double max_div_impl(IBMSolver *s);
+
+// timing helpers (step.cpp)
+int timer_begin(IBMSolver *s, const char *name, double bytes);
+void timer_end(IBMSolver *s, int id);
+void timers_flush(IBMSolver *s);
+
+// geo3d.cpp
+void polyhedron_cpt_grid(const Polyhedron *poly, int nx, int ny, int nz, const double *x, const double *y,
+                         const double *z, double *f, double maxd);
+
+// export.cpp
+void export_result_impl(IBMSolver *s, const char *filename);
+void export_lvset_flag_impl(IBMSolver *s, const char *filename);
+void export_intermediate_impl(IBMSolver *s, const char *filename);
+void init_flow_file_impl(IBMSolver *s, const char *filename);
+
+}  // namespace ibm

@@ -1,0 +1,100 @@
+// solver_kernels.h - matrix-free operators, smoothers, Krylov vector kernels
+// and multigrid transfer kernels (templated on FP64 / FP32).
+#pragma once
+
+#include "common.h"
+
+namespace ibm {
+
+// Ghost rows of one system in compact form (struct of arrays).
+template <typename T>
+struct GhostRows {
+    int ng;
+    const long *cell;   // ng
+    const long *src;    // 8 * ng (-1 = unused)
+    const T *w;         // 8 * ng
+    const T *wself;     // ng
+};
+
+// Device-resident scalars of BiCGSTAB.
+struct KScal {
+    double rho, rho1, alpha, omega, beta;
+    double rr, rrn, bb, bbn;
+    double ts, tt, rv;
+    int breakdown;
+    int pad;
+};
+
+// Which per-row scale the normalised residual uses: none, or 1 / sum K on
+// fluid rows (the reference's diagonal-normalised pressure rows).
+struct NormSpec {
+    int on;
+    const double *ksx, *ksy, *ksz;  // [n] sums of the unfolded K per axis
+    const uint8_t *flag;
+    long pin;
+};
+
+template <typename T>
+void launch_apply(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
+                  const T *x, T *y, int mode, const T *w, double *partial);
+long apply_partials(const Dims &d, int ng);
+
+template <typename T>
+void launch_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G,
+                     const uint8_t *flag, const T *b, const T *x, T *r, const NormSpec &ns, double *partial);
+
+template <typename T>
+void launch_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
+                   const T *r, T *z);
+
+template <typename T>
+void launch_rbgs(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
+                 const T *b, T *x, int sweeps);
+
+// BiCGSTAB pieces
+template <typename T>
+void launch_bicg_init(hipStream_t st, const Dims &d, T *r, T *rh, T *p, T *v, T *x, const T *b);
+template <typename T>
+void launch_bicg_p(hipStream_t st, const Dims &d, const KScal *ks, const T *r, const T *v, T *p);
+template <typename T>
+void launch_bicg_s(hipStream_t st, const Dims &d, const KScal *ks, const T *r, const T *v, T *s);
+template <typename T>
+void launch_bicg_xr(hipStream_t st, const Dims &d, const KScal *ks, const T *ph, const T *sh, const T *s,
+                    const T *t, const T *rh, T *x, T *r, const NormSpec &ns, double *partial);
+long vec_partials(const Dims &d);
+
+// finalize ops: 0 = alpha from dot(rh, v); 1 = omega from (t.s, t.t);
+// 2 = rho/beta/norms after the x, r update; 3 = store b norms and rho1 at
+// start; 4 = store residual norms only
+void launch_finalize(hipStream_t st, int op, const double *partial, long nblk, int nv, KScal *ks);
+// out[a] = sum (or max) over blocks of partial[b * nv + a]
+void launch_sum_partials(hipStream_t st, const double *partial, long nblk, int nv, double *out, int reduce_max);
+
+// precision conversion / accumulation over interior cells
+void launch_d2f(hipStream_t st, const Dims &d, const double *a, float *b);
+void launch_add_f2d(hipStream_t st, const Dims &d, const float *e, double *x);
+template <typename T>
+void launch_zero(hipStream_t st, const Dims &d, T *a);
+template <typename T>
+void launch_copy(hipStream_t st, const Dims &d, const T *a, T *b);
+
+// multigrid transfers between a fine grid df and a coarse grid dc
+struct MGTransfer {
+    int cx, cy, cz;               // 1 if the axis is coarsened by 2
+    const int *pi, *pj, *pk;      // fine index -> far coarse neighbour index (or same)
+    const double *wi, *wj, *wk;   // weight of the near coarse cell (1 - weight of far)
+    const double *vol_f;          // unused placeholder
+};
+
+template <typename T>
+void launch_restrict(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
+                     const T *rf, const double *dxf, const double *dyf, const double *dzf, const double *dxc,
+                     const double *dyc, const double *dzc, T *bc);
+template <typename T>
+void launch_prolong(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
+                    const uint8_t *flag_c, const T *ec, T *xf);
+template <typename T>
+void launch_coarse_solve(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b, T *x,
+                         int sweeps, int singular, const double *dxyz);
+
+}  // namespace ibm

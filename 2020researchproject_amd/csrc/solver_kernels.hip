@@ -1,0 +1,663 @@
+// solver_kernels.hip - linear-algebra kernels of the momentum and pressure
+// solves (the part the reference delegates to Hypre: src/ibm3d_fracstep.c:
+// 354-379 and 649-703, matrices of src/ibm3d_setup.c:1566-2908).
+//
+// The operator is matrix-free: fluid rows are the 7-point stencil with
+// separable per-axis coefficients (the reference's kx_W .. kz_U, with the
+// outer boundary rows folded into the first/last interior cells), ghost rows
+// come from a compact list, solid and pinned rows are the identity. Vectors
+// live in the padded cell layout of common.h and only interior cells are
+// touched. Reductions are deterministic: per-block partial sums in a fixed
+// order, then a one-block finalize kernel.
+#include "solver_kernels.h"
+
+namespace ibm {
+
+#define C1(a, i) ((a)[(i) + 2])
+
+__device__ inline long blin() {
+    return blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+}
+
+template <int NV>
+__device__ inline void bsum(const double *v, double *partial, long blk) {
+    __shared__ double sh[NV][16];
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x + threadIdx.z * blockDim.x * blockDim.y;
+    const int nth = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int a = 0; a < NV; a++) {
+        double x = v[a];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (lane == 0) sh[a][wave] = x;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = (nth + 63) >> 6;
+#pragma unroll
+        for (int a = 0; a < NV; a++) {
+            double acc = 0;
+            for (int w = 0; w < nw; w++) acc += sh[a][w];
+            partial[blk * NV + a] = acc;
+        }
+    }
+}
+
+static inline dim3 igrid(const Dims &d) { return dim3((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz); }
+static inline long iblocks(const Dims &d) { return (long)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz; }
+
+// Fluid-row stencil: returns diag and the off-diagonal sum sum(c x_nb).
+template <typename T>
+__device__ inline void stencil(const Dims &d, const SysCoef<T> &S, const uint8_t *flag, int i, int j, int k,
+                               long q, const T *x, T &diag, T &off) {
+    diag = S.c0;
+    off = 0;
+    const int id[3] = {i, j, k};
+    const long st[3] = {1, (long)d.sy, d.sz};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const AxisCoef<T> &A = S.ax[a];
+        const int n = A.n, ii = id[a];
+        const long qm = (ii == 0 && A.periodic) ? q + (long)(n - 1) * st[a] : q - st[a];
+        const long qp = (ii == n - 1 && A.periodic) ? q - (long)(n - 1) * st[a] : q + st[a];
+        T cm = A.cm[ii], cp = A.cp[ii], dm = A.dm[ii], dp = A.dp[ii];
+        if (S.skip) {
+            if ((ii > 0 || A.periodic) && flag[qm] == F_SOLID) { cm = 0; dm = 0; }
+            if ((ii < n - 1 || A.periodic) && flag[qp] == F_SOLID) { cp = 0; dp = 0; }
+        }
+        diag += dm + dp;
+        off += cm * x[qm] + cp * x[qp];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y = A x (interior non-ghost rows); mode 1 adds dot(w, y), mode 2 adds
+// dot(y, w) and dot(y, y).
+// ---------------------------------------------------------------------------
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_apply(Dims d, SysCoef<T> S, const uint8_t *flag, const T *x, T *y,
+                                               const T *w, double *partial) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    double acc[2] = {0, 0};
+    if (i < d.nx && j < d.ny) {
+        const long q = cidx(d, i, j, k);
+        const uint8_t f = flag[q];
+        bool have = true;
+        T yv;
+        if (q == S.pin || f == F_SOLID) {
+            yv = x[q];
+        } else if (f == F_FLUID) {
+            T diag, off;
+            stencil(d, S, flag, i, j, k, q, x, diag, off);
+            yv = diag * x[q] - off;
+        } else {
+            have = false;
+        }
+        if (have) {
+            y[q] = yv;
+            if (MODE == 1) acc[0] = (double)w[q] * (double)yv;
+            if (MODE == 2) { acc[0] = (double)yv * (double)w[q]; acc[1] = (double)yv * (double)yv; }
+        }
+    }
+    if constexpr (MODE > 0) bsum<MODE>(acc, partial, blin());
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_apply_ghost(GhostRows<T> G, int is_p, const T *x, T *y, const T *w,
+                                                     double *partial, long blk0) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    double acc[2] = {0, 0};
+    if (s < G.ng) {
+        T a = 0;
+#pragma unroll
+        for (int l = 0; l < 8; l++) {
+            const long c = G.src[8 * (long)s + l];
+            if (c >= 0) a += G.w[8 * (long)s + l] * x[c];
+        }
+        const long c = G.cell[s];
+        const T ws = G.wself[s];
+        const T yv = is_p ? (T(1) - ws) * x[c] - a : (T(1) + ws) * x[c] + a;
+        y[c] = yv;
+        if (MODE == 1) acc[0] = (double)w[c] * (double)yv;
+        if (MODE == 2) { acc[0] = (double)yv * (double)w[c]; acc[1] = (double)yv * (double)yv; }
+    }
+    if constexpr (MODE > 0) bsum<MODE>(acc, partial, blk0 + blockIdx.x);
+}
+
+long apply_partials(const Dims &d, int ng) { return iblocks(d) + (ng + 255) / 256; }
+
+template <typename T>
+void launch_apply(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
+                  const T *x, T *y, int mode, const T *w, double *partial) {
+    const long nb = iblocks(d);
+    const int gb = (G.ng + 255) / 256;
+    if (mode == 0) {
+        hipLaunchKernelGGL((k_apply<T, 0>), igrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
+        if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 0>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
+    } else if (mode == 1) {
+        hipLaunchKernelGGL((k_apply<T, 1>), igrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
+        if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 1>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
+    } else {
+        hipLaunchKernelGGL((k_apply<T, 2>), igrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
+        if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 2>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// r = b - A x with partial norms ||r||^2 and ||D^-1 r||^2 (D = sum K on fluid
+// rows when ns.on, else identity).
+// ---------------------------------------------------------------------------
+__device__ inline double nscale(const NormSpec &ns, const Dims &d, int i, int j, int k, long q) {
+    if (!ns.on || q == ns.pin || ns.flag[q] != F_FLUID) return 1.0;
+    return 1.0 / (ns.ksx[i] + ns.ksy[j] + ns.ksz[k]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_residual(Dims d, SysCoef<T> S, const uint8_t *flag, const T *b,
+                                                  const T *x, T *r, NormSpec ns, double *partial) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    double acc[2] = {0, 0};
+    if (i < d.nx && j < d.ny) {
+        const long q = cidx(d, i, j, k);
+        const uint8_t f = flag[q];
+        bool have = true;
+        T rv;
+        if (q == S.pin || f == F_SOLID) {
+            rv = b[q] - x[q];
+        } else if (f == F_FLUID) {
+            T diag, off;
+            stencil(d, S, flag, i, j, k, q, x, diag, off);
+            rv = b[q] - (diag * x[q] - off);
+        } else {
+            have = false;
+        }
+        if (have) {
+            r[q] = rv;
+            const double rd = (double)rv;
+            acc[0] = rd * rd;
+            const double rs = rd * nscale(ns, d, i, j, k, q);
+            acc[1] = rs * rs;
+        }
+    }
+    bsum<2>(acc, partial, blin());
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_residual_ghost(GhostRows<T> G, int is_p, const T *b, const T *x, T *r,
+                                                        double *partial, long blk0) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    double acc[2] = {0, 0};
+    if (s < G.ng) {
+        T a = 0;
+#pragma unroll
+        for (int l = 0; l < 8; l++) {
+            const long c = G.src[8 * (long)s + l];
+            if (c >= 0) a += G.w[8 * (long)s + l] * x[c];
+        }
+        const long c = G.cell[s];
+        const T ws = G.wself[s];
+        const T yv = is_p ? (T(1) - ws) * x[c] - a : (T(1) + ws) * x[c] + a;
+        const T rv = b[c] - yv;
+        r[c] = rv;
+        acc[0] = acc[1] = (double)rv * (double)rv;
+    }
+    bsum<2>(acc, partial, blk0 + blockIdx.x);
+}
+
+template <typename T>
+void launch_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G,
+                     const uint8_t *flag, const T *b, const T *x, T *r, const NormSpec &ns, double *partial) {
+    const long nb = iblocks(d);
+    hipLaunchKernelGGL((k_residual<T>), igrid(d), dim3(64, 4), 0, st, d, S, flag, b, x, r, ns, partial);
+    const int gb = (G.ng + 255) / 256;
+    if (gb) hipLaunchKernelGGL((k_residual_ghost<T>), dim3(gb), dim3(256), 0, st, G, S.is_p, b, x, r, partial, nb);
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi preconditioner z = D^-1 r
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_jacobi(Dims d, SysCoef<T> S, const uint8_t *flag, const T *r, T *z) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    const uint8_t f = flag[q];
+    if (q == S.pin || f == F_SOLID) { z[q] = r[q]; return; }
+    if (f != F_FLUID) return;
+    T diag = S.c0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int ii = a == 0 ? i : a == 1 ? j : k;
+        diag += S.ax[a].dm[ii] + S.ax[a].dp[ii];
+    }
+    z[q] = diag != T(0) ? r[q] / diag : T(0);
+}
+
+template <typename T>
+__global__ void k_jacobi_ghost(GhostRows<T> G, int is_p, const T *r, T *z) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= G.ng) return;
+    const long c = G.cell[s];
+    const T dg = is_p ? T(1) - G.wself[s] : T(1) + G.wself[s];
+    z[c] = dg != T(0) ? r[c] / dg : r[c];
+}
+
+template <typename T>
+void launch_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
+                   const T *r, T *z) {
+    hipLaunchKernelGGL((k_jacobi<T>), igrid(d), dim3(64, 4), 0, st, d, S, flag, r, z);
+    const int gb = (G.ng + 255) / 256;
+    if (gb) hipLaunchKernelGGL((k_jacobi_ghost<T>), dim3(gb), dim3(256), 0, st, G, S.is_p, r, z);
+}
+
+// ---------------------------------------------------------------------------
+// Red-black Gauss-Seidel on fluid rows, Jacobi on ghost rows after each colour.
+// Thread t of a row handles x = 2t + parity so every lane owns one cell.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_rbgs(Dims d, SysCoef<T> S, const uint8_t *flag, const T *b, T *x,
+                                              int color) {
+    const int j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int i = 2 * (blockIdx.x * 64 + threadIdx.x) + ((j + k + color) & 1);
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    const uint8_t f = flag[q];
+    if (q == S.pin || f == F_SOLID) { x[q] = b[q]; return; }
+    if (f != F_FLUID) return;
+    T diag, off;
+    stencil(d, S, flag, i, j, k, q, x, diag, off);
+    x[q] = diag != T(0) ? (b[q] + off) / diag : T(0);
+}
+
+template <typename T>
+__global__ void k_ghost_relax(GhostRows<T> G, int is_p, const T *b, T *x) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= G.ng) return;
+    T a = 0;
+#pragma unroll
+    for (int l = 0; l < 8; l++) {
+        const long c = G.src[8 * (long)s + l];
+        if (c >= 0) a += G.w[8 * (long)s + l] * x[c];
+    }
+    const long c = G.cell[s];
+    const T ws = G.wself[s];
+    if (is_p) {
+        const T dg = T(1) - ws;
+        if (dg != T(0)) x[c] = (b[c] + a) / dg;
+    } else {
+        x[c] = (b[c] - a) / (T(1) + ws);
+    }
+}
+
+template <typename T>
+void launch_rbgs(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
+                 const T *b, T *x, int sweeps) {
+    dim3 grid(((d.nx + 1) / 2 + 63) / 64, (d.ny + 3) / 4, d.nz);
+    const int gb = (G.ng + 255) / 256;
+    for (int s = 0; s < sweeps; s++)
+        for (int c = 0; c < 2; c++) {
+            hipLaunchKernelGGL((k_rbgs<T>), grid, dim3(64, 4), 0, st, d, S, flag, b, x, c);
+            if (gb) hipLaunchKernelGGL((k_ghost_relax<T>), dim3(gb), dim3(256), 0, st, G, S.is_p, b, x);
+        }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB vector kernels (van der Vorst 1992) with device-resident scalars.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_bicg_init(Dims d, T *r, T *rh, T *p, T *v, T *x, const T *b) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    const T bv = b[q];
+    r[q] = bv; rh[q] = bv; p[q] = 0; v[q] = 0; x[q] = 0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bicg_p(Dims d, const KScal *ks, const T *r, const T *v, T *p) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    const T beta = (T)ks->beta, omega = (T)ks->omega;
+    p[q] = r[q] + beta * (p[q] - omega * v[q]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bicg_s(Dims d, const KScal *ks, const T *r, const T *v, T *s) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    s[q] = r[q] - (T)ks->alpha * v[q];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bicg_xr(Dims d, const KScal *ks, const T *ph, const T *sh, const T *s,
+                                                 const T *t, const T *rh, T *x, T *r, NormSpec ns, double *partial) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    double acc[3] = {0, 0, 0};
+    if (i < d.nx && j < d.ny) {
+        const long q = cidx(d, i, j, k);
+        const T alpha = (T)ks->alpha, omega = (T)ks->omega;
+        x[q] = x[q] + alpha * ph[q] + omega * sh[q];
+        const T rv = s[q] - omega * t[q];
+        r[q] = rv;
+        const double rd = (double)rv;
+        acc[0] = rd * rd;
+        const double rs = rd * nscale(ns, d, i, j, k, q);
+        acc[1] = rs * rs;
+        acc[2] = (double)rh[q] * rd;
+    }
+    bsum<3>(acc, partial, blin());
+}
+
+long vec_partials(const Dims &d) { return iblocks(d); }
+
+template <typename T>
+void launch_bicg_init(hipStream_t st, const Dims &d, T *r, T *rh, T *p, T *v, T *x, const T *b) {
+    hipLaunchKernelGGL((k_bicg_init<T>), igrid(d), dim3(64, 4), 0, st, d, r, rh, p, v, x, b);
+}
+template <typename T>
+void launch_bicg_p(hipStream_t st, const Dims &d, const KScal *ks, const T *r, const T *v, T *p) {
+    hipLaunchKernelGGL((k_bicg_p<T>), igrid(d), dim3(64, 4), 0, st, d, ks, r, v, p);
+}
+template <typename T>
+void launch_bicg_s(hipStream_t st, const Dims &d, const KScal *ks, const T *r, const T *v, T *s) {
+    hipLaunchKernelGGL((k_bicg_s<T>), igrid(d), dim3(64, 4), 0, st, d, ks, r, v, s);
+}
+template <typename T>
+void launch_bicg_xr(hipStream_t st, const Dims &d, const KScal *ks, const T *ph, const T *sh, const T *s,
+                    const T *t, const T *rh, T *x, T *r, const NormSpec &ns, double *partial) {
+    hipLaunchKernelGGL((k_bicg_xr<T>), igrid(d), dim3(64, 4), 0, st, d, ks, ph, sh, s, t, rh, x, r, ns, partial);
+}
+
+// one block of 256 threads; thread t sums partials t, t+256, ... in order,
+// then a fixed-shape tree: bitwise reproducible for a given grid.
+__global__ void __launch_bounds__(256) k_finalize(int op, const double *partial, long nblk, int nv, KScal *ks) {
+    __shared__ double sh[4][256];
+    const int t = threadIdx.x;
+    double acc[4] = {0, 0, 0, 0};
+    for (long b = t; b < nblk; b += 256)
+        for (int a = 0; a < nv; a++) acc[a] += partial[b * nv + a];
+    for (int a = 0; a < nv; a++) sh[a][t] = acc[a];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s)
+            for (int a = 0; a < nv; a++) sh[a][t] += sh[a][t + s];
+        __syncthreads();
+    }
+    if (t != 0) return;
+    const double s0 = sh[0][0], s1 = nv > 1 ? sh[1][0] : 0, s2 = nv > 2 ? sh[2][0] : 0;
+    switch (op) {
+    case 0:  // alpha = rho1 / (rh . v)
+        ks->rv = s0;
+        if (s0 != 0) ks->alpha = ks->rho1 / s0;
+        else { ks->alpha = 0; ks->breakdown = 1; }
+        break;
+    case 1:  // omega = (t . s) / (t . t)
+        ks->ts = s0; ks->tt = s1;
+        ks->omega = s1 > 0 ? s0 / s1 : 0;
+        break;
+    case 2: {  // after x, r update
+        ks->rr = s0; ks->rrn = s1;
+        const double rho1 = s2;
+        ks->rho = ks->rho1;
+        ks->rho1 = rho1;
+        if (ks->rho != 0 && ks->omega != 0) ks->beta = (rho1 / ks->rho) * (ks->alpha / ks->omega);
+        else { ks->beta = 0; ks->breakdown = 1; }
+        break;
+    }
+    case 3:  // start: norms of b (= r0 = rhat), rho1 = (rhat, r0)
+        ks->bb = s0; ks->bbn = s1; ks->rr = s0; ks->rrn = s1;
+        ks->rho = 1; ks->alpha = 1; ks->omega = 1;
+        ks->rho1 = s0;
+        ks->beta = 0;
+        ks->breakdown = 0;
+        break;
+    case 4:  // residual norms only
+        ks->rr = s0; ks->rrn = s1;
+        break;
+    default:;
+    }
+}
+
+// sums nv partial columns into out[0..nv) (same fixed order as k_finalize)
+__global__ void __launch_bounds__(256) k_sum_partials(const double *partial, long nblk, int nv, double *out, int reduce_max) {
+    __shared__ double sh[4][256];
+    const int t = threadIdx.x;
+    double acc[4] = {0, 0, 0, 0};
+    for (long b = t; b < nblk; b += 256)
+        for (int a = 0; a < nv; a++) acc[a] = reduce_max ? fmax(acc[a], partial[b * nv + a]) : acc[a] + partial[b * nv + a];
+    for (int a = 0; a < nv; a++) sh[a][t] = acc[a];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s)
+            for (int a = 0; a < nv; a++) sh[a][t] = reduce_max ? fmax(sh[a][t], sh[a][t + s]) : sh[a][t] + sh[a][t + s];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int a = 0; a < nv; a++) out[a] = sh[a][0];
+}
+
+void launch_sum_partials(hipStream_t st, const double *partial, long nblk, int nv, double *out, int reduce_max) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partial, nblk, nv, out, reduce_max);
+}
+
+void launch_finalize(hipStream_t st, int op, const double *partial, long nblk, int nv, KScal *ks) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, op, partial, nblk, nv, ks);
+}
+
+// ---------------------------------------------------------------------------
+// precision conversion and copies over interior cells
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_d2f(Dims d, const double *a, float *b) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    b[q] = (float)a[q];
+}
+__global__ void __launch_bounds__(256) k_add_f2d(Dims d, const float *e, double *x) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    x[q] += (double)e[q];
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_zero(Dims d, T *a) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    a[cidx(d, i, j, k)] = 0;
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_copy(Dims d, const T *a, T *b) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    b[q] = a[q];
+}
+void launch_d2f(hipStream_t st, const Dims &d, const double *a, float *b) {
+    hipLaunchKernelGGL(k_d2f, igrid(d), dim3(64, 4), 0, st, d, a, b);
+}
+void launch_add_f2d(hipStream_t st, const Dims &d, const float *e, double *x) {
+    hipLaunchKernelGGL(k_add_f2d, igrid(d), dim3(64, 4), 0, st, d, e, x);
+}
+template <typename T>
+void launch_zero(hipStream_t st, const Dims &d, T *a) {
+    hipLaunchKernelGGL((k_zero<T>), igrid(d), dim3(64, 4), 0, st, d, a);
+}
+template <typename T>
+void launch_copy(hipStream_t st, const Dims &d, const T *a, T *b) {
+    hipLaunchKernelGGL((k_copy<T>), igrid(d), dim3(64, 4), 0, st, d, a, b);
+}
+
+// ---------------------------------------------------------------------------
+// multigrid transfers
+// ---------------------------------------------------------------------------
+// coarse rhs = volume-weighted sum of the fluid children's residuals / coarse volume
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict(Dims df, Dims dc, MGTransfer tr, const uint8_t *flag_f,
+                                                  const T *rf, const double *dxf, const double *dyf,
+                                                  const double *dzf, const double *dxc, const double *dyc,
+                                                  const double *dzc, T *bc) {
+    const int I = blockIdx.x * 64 + threadIdx.x, J = blockIdx.y * 4 + threadIdx.y, K = blockIdx.z;
+    if (I >= dc.nx || J >= dc.ny) return;
+    double acc = 0;
+    const int i0 = tr.cx ? 2 * I : I, j0 = tr.cy ? 2 * J : J, k0 = tr.cz ? 2 * K : K;
+    for (int c = 0; c <= tr.cz; c++)
+        for (int b = 0; b <= tr.cy; b++)
+            for (int a = 0; a <= tr.cx; a++) {
+                const int i = i0 + a, j = j0 + b, k = k0 + c;
+                const long q = cidx(df, i, j, k);
+                if (flag_f[q] != F_FLUID) continue;
+                acc += (double)rf[q] * C1(dxf, i) * C1(dyf, j) * C1(dzf, k);
+            }
+    bc[cidx(dc, I, J, K)] = (T)(acc / (C1(dxc, I) * C1(dyc, J) * C1(dzc, K)));
+}
+
+// x_fine += trilinear interpolation of the coarse correction (fluid cells)
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong(Dims df, Dims dc, MGTransfer tr, const uint8_t *flag_f,
+                                                 const uint8_t *flag_c, const T *ec, T *xf) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= df.nx || j >= df.ny) return;
+    const long q = cidx(df, i, j, k);
+    if (flag_f[q] != F_FLUID) return;
+    const int I = tr.cx ? i >> 1 : i, J = tr.cy ? j >> 1 : j, K = tr.cz ? k >> 1 : k;
+    const int Is[2] = {I, tr.pi[i]}, Js[2] = {J, tr.pj[j]}, Ks[2] = {K, tr.pk[k]};
+    const double wi[2] = {tr.wi[i], 1 - tr.wi[i]}, wj[2] = {tr.wj[j], 1 - tr.wj[j]}, wk[2] = {tr.wk[k], 1 - tr.wk[k]};
+    const long q0 = cidx(dc, I, J, K);
+    const double e0 = (double)ec[q0];
+    double e = 0;
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const double w = wi[a] * wj[b] * wk[c];
+                if (w == 0) continue;
+                const long qc = cidx(dc, Is[a], Js[b], Ks[c]);
+                e += w * (flag_c[qc] == F_SOLID ? e0 : (double)ec[qc]);
+            }
+    xf[q] += (T)e;
+}
+
+// single-workgroup red-black Gauss-Seidel on the coarsest grid; for a
+// singular (all-Neumann) problem the volume-weighted mean of b is removed
+// first and of x afterwards.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_coarse(Dims d, SysCoef<T> S, const uint8_t *flag, const T *b, T *x,
+                                                 int sweeps, int singular, const double *dx, const double *dy,
+                                                 const double *dz) {
+    __shared__ double red[2][1024];
+    const int t = threadIdx.x;
+    const long n = (long)d.nx * d.ny * d.nz;
+    double bmean = 0;
+    if (singular) {
+        double sv = 0, sb = 0;
+        for (long c = t; c < n; c += 1024) {
+            const int i = c % d.nx, j = (c / d.nx) % d.ny, k = c / ((long)d.nx * d.ny);
+            const long q = cidx(d, i, j, k);
+            if (flag[q] != F_FLUID) continue;
+            const double v = C1(dx, i) * C1(dy, j) * C1(dz, k);
+            sv += v; sb += v * (double)b[q];
+        }
+        red[0][t] = sv; red[1][t] = sb;
+        __syncthreads();
+        for (int s = 512; s > 0; s >>= 1) {
+            if (t < s) { red[0][t] += red[0][t + s]; red[1][t] += red[1][t + s]; }
+            __syncthreads();
+        }
+        bmean = red[0][0] > 0 ? red[1][0] / red[0][0] : 0;
+        __syncthreads();
+    }
+    for (long c = t; c < n; c += 1024) {
+        const int i = c % d.nx, j = (c / d.nx) % d.ny, k = c / ((long)d.nx * d.ny);
+        x[cidx(d, i, j, k)] = 0;
+    }
+    __syncthreads();
+    for (int s = 0; s < sweeps; s++)
+        for (int color = 0; color < 2; color++) {
+            for (long c = t; c < n; c += 1024) {
+                const int i = c % d.nx, j = (c / d.nx) % d.ny, k = c / ((long)d.nx * d.ny);
+                if (((i + j + k) & 1) != color) continue;
+                const long q = cidx(d, i, j, k);
+                const uint8_t f = flag[q];
+                if (q == S.pin || f == F_SOLID) { x[q] = b[q]; continue; }
+                T diag, off;
+                stencil(d, S, flag, i, j, k, q, x, diag, off);
+                const T rhs = b[q] - (T)bmean;
+                x[q] = diag != T(0) ? (rhs + off) / diag : T(0);
+            }
+            __syncthreads();
+        }
+    if (singular) {
+        double sv = 0, sx = 0;
+        for (long c = t; c < n; c += 1024) {
+            const int i = c % d.nx, j = (c / d.nx) % d.ny, k = c / ((long)d.nx * d.ny);
+            const long q = cidx(d, i, j, k);
+            if (flag[q] != F_FLUID) continue;
+            const double v = C1(dx, i) * C1(dy, j) * C1(dz, k);
+            sv += v; sx += v * (double)x[q];
+        }
+        red[0][t] = sv; red[1][t] = sx;
+        __syncthreads();
+        for (int s = 512; s > 0; s >>= 1) {
+            if (t < s) { red[0][t] += red[0][t + s]; red[1][t] += red[1][t + s]; }
+            __syncthreads();
+        }
+        const double xm = red[0][0] > 0 ? red[1][0] / red[0][0] : 0;
+        for (long c = t; c < n; c += 1024) {
+            const int i = c % d.nx, j = (c / d.nx) % d.ny, k = c / ((long)d.nx * d.ny);
+            const long q = cidx(d, i, j, k);
+            if (flag[q] == F_FLUID) x[q] -= (T)xm;
+        }
+    }
+}
+
+template <typename T>
+void launch_restrict(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
+                     const T *rf, const double *dxf, const double *dyf, const double *dzf, const double *dxc,
+                     const double *dyc, const double *dzc, T *bc) {
+    hipLaunchKernelGGL((k_restrict<T>), igrid(dc), dim3(64, 4), 0, st, df, dc, tr, flag_f, rf, dxf, dyf, dzf, dxc, dyc,
+                       dzc, bc);
+}
+template <typename T>
+void launch_prolong(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
+                    const uint8_t *flag_c, const T *ec, T *xf) {
+    hipLaunchKernelGGL((k_prolong<T>), igrid(df), dim3(64, 4), 0, st, df, dc, tr, flag_f, flag_c, ec, xf);
+}
+template <typename T>
+void launch_coarse_solve(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b, T *x,
+                         int sweeps, int singular, const double *dxyz) {
+    const double *dx = dxyz, *dy = dxyz + d.nx + 4, *dz = dxyz + d.nx + 4 + d.ny + 4;
+    hipLaunchKernelGGL((k_coarse<T>), dim3(1), dim3(1024), 0, st, d, S, flag, b, x, sweeps, singular, dx, dy, dz);
+}
+
+// explicit instantiations
+#define INST(T)                                                                                                   \
+    template void launch_apply<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,             \
+                                  const uint8_t *, const T *, T *, int, const T *, double *);                      \
+    template void launch_residual<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,          \
+                                     const uint8_t *, const T *, const T *, T *, const NormSpec &, double *);       \
+    template void launch_jacobi<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,            \
+                                   const uint8_t *, const T *, T *);                                               \
+    template void launch_rbgs<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,              \
+                                 const uint8_t *, const T *, T *, int);                                            \
+    template void launch_bicg_init<T>(hipStream_t, const Dims &, T *, T *, T *, T *, T *, const T *);              \
+    template void launch_bicg_p<T>(hipStream_t, const Dims &, const KScal *, const T *, const T *, T *);           \
+    template void launch_bicg_s<T>(hipStream_t, const Dims &, const KScal *, const T *, const T *, T *);           \
+    template void launch_bicg_xr<T>(hipStream_t, const Dims &, const KScal *, const T *, const T *, const T *,     \
+                                    const T *, const T *, T *, T *, const NormSpec &, double *);                   \
+    template void launch_zero<T>(hipStream_t, const Dims &, T *);                                                  \
+    template void launch_copy<T>(hipStream_t, const Dims &, const T *, T *);                                       \
+    template void launch_restrict<T>(hipStream_t, const Dims &, const Dims &, const MGTransfer &, const uint8_t *, \
+                                     const T *, const double *, const double *, const double *, const double *,    \
+                                     const double *, const double *, T *);                                         \
+    template void launch_prolong<T>(hipStream_t, const Dims &, const Dims &, const MGTransfer &, const uint8_t *,  \
+                                    const uint8_t *, const T *, T *);                                              \
+    template void launch_coarse_solve<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *,           \
+                                         const T *, T *, int, int, const double *);
+INST(double)
+INST(float)
+
+}  // namespace ibm

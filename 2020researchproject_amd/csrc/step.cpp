@@ -1,0 +1,696 @@
+// step.cpp - the time loop (IBMSolver_iterate) and the linear solves.
+//
+// Stage order is the reference's (src/ibm3d_fracstep.c:38-115). The Hypre
+// BiCGSTAB / BoomerAMG solves (:354-379, :649-703) are replaced by
+//   FP64 mode : BiCGSTAB with a Jacobi preconditioner (momentum) or a
+//               geometric-multigrid V-cycle (pressure), FP64 throughout;
+//   mixed mode: FP64 iterative refinement (defect correction on the full
+//               operator, FP64 residuals and solution) around FP32 BiCGSTAB
+//               inner solves preconditioned by Jacobi / an FP32 V-cycle.
+// Both stop on the reference's criterion ||b - A x||_2 <= tol ||b||_2 of the
+// full padded system (pressure rows diagonal-normalised), from x0 = 0.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "solver.h"
+
+namespace ibm {
+
+#define H1(a, i) ((a)[(i) + 2])
+
+static bool isperiodic(IBMSolverBCType t) { return t == BC_ALL_PERIODIC || t == BC_VELOCITY_PERIODIC; }
+static int face_bc(int ax, int hi) {
+    static const int t[3][2] = {{3, 1}, {2, 0}, {4, 5}};
+    return t[ax][hi];
+}
+
+// ---------------------------------------------------------------------------
+// timing
+// ---------------------------------------------------------------------------
+static hipEvent_t get_event(IBMSolver *s) {
+    if (!s->event_pool.empty()) {
+        hipEvent_t e = s->event_pool.back();
+        s->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+int timer_begin(IBMSolver *s, const char *name, double bytes) {
+    if (!s->timing) return -1;
+    int id = -1;
+    for (size_t t = 0; t < s->timers.size(); t++)
+        if (s->timers[t].name == name) { id = (int)t; break; }
+    if (id < 0) {
+        s->timers.emplace_back();
+        id = (int)s->timers.size() - 1;
+        s->timers[id].name = name;
+    }
+    s->timers[id].bytes = bytes;
+    hipEvent_t a = get_event(s), b = get_event(s);
+    HIP_CHECK(hipEventRecord(a, s->stream));
+    s->timers[id].pending.push_back({a, b});
+    return id;
+}
+
+void timer_end(IBMSolver *s, int id) {
+    if (id < 0) return;
+    HIP_CHECK(hipEventRecord(s->timers[id].pending.back().second, s->stream));
+}
+
+void timers_flush(IBMSolver *s) {
+    for (auto &t : s->timers) {
+        for (auto &e : t.pending) {
+            HIP_CHECK(hipEventSynchronize(e.second));
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+            t.total_ms += ms;
+            t.count++;
+            s->event_pool.push_back(e.first);
+            s->event_pool.push_back(e.second);
+        }
+        t.pending.clear();
+    }
+}
+
+#define TIMED(name, bytes, ...)                       \
+    do {                                              \
+        const int tid_ = timer_begin(s, name, bytes); \
+        __VA_ARGS__;                                  \
+        timer_end(s, tid_);                           \
+    } while (0)
+
+static double ncells(const IBMSolver *s) { return (double)s->Nx * s->Ny * s->Nz; }
+
+// ---------------------------------------------------------------------------
+// boundary data
+// ---------------------------------------------------------------------------
+enum { BCD_U = 0, BCD_P = 1, BCD_USOL = 2, BCD_PSOL = 3 };
+
+// Builds the face data for one use: velocity at time t (BCD_U, BCD_USOL),
+// pressure at t (BCD_P) or the increment p_bc(t + dt) - p_bc(t) (BCD_PSOL).
+// VAL_FUNC faces are evaluated on the host over the reference's tangential
+// ranges and uploaded; norm2[c] returns 2 sum g^2 of their BC rows.
+static OuterParams make_outer(IBMSolver *s, int what, double t, double norm2[4]) {
+    OuterParams op{};
+    if (norm2) for (int c = 0; c < 4; c++) norm2[c] = 0;
+    for (int ax = 0; ax < 3; ax++)
+        for (int hi = 0; hi < 2; hi++) {
+            const HostBC &B = s->bc[face_bc(ax, hi)];
+            FaceBC &F = op.face[ax][hi];
+            F.type = (int)B.type;
+            int n1, n2, o1, o2;
+            if (ax == 0) { n1 = s->Ny; n2 = s->Nz; o1 = 0; o2 = 0; }
+            else if (ax == 1) { n1 = s->Nx + 4; n2 = s->Nz; o1 = 2; o2 = 0; }
+            else { n1 = s->Nx + 4; n2 = s->Ny + 4; o1 = 2; o2 = 2; }
+            F.n1 = n1; F.t1off = o1; F.t2off = o2;
+            for (int c = 0; c < 3; c++) F.g[c] = B.cu[c];
+            F.g[3] = what == BCD_PSOL ? 0.0 : B.cp;
+            F.garr = nullptr;
+            if (B.val_type != VAL_FUNC) continue;
+            std::vector<double> h(4 * (size_t)n1 * n2, 0.0);
+            for (int a = 0; a < n1; a++)
+                for (int b = 0; b < n2; b++) {
+                    const int t1 = a - o1, t2 = b - o2;
+                    double x, y, z;
+                    if (ax == 0) { x = hi ? s->xmax : s->xmin; y = H1(s->yc, t1); z = H1(s->zc, t2); }
+                    else if (ax == 1) { x = H1(s->xc, t1); y = hi ? s->ymax : s->ymin; z = H1(s->zc, t2); }
+                    else { x = H1(s->xc, t1); y = H1(s->yc, t2); z = hi ? s->zmax : s->zmin; }
+                    const size_t o = (size_t)a * n2 + b;
+                    if (B.type == BC_VELOCITY_COMPONENT)
+                        for (int c = 0; c < 3; c++) h[c * (size_t)n1 * n2 + o] = B.fu[c](t, x, y, z);
+                    if (B.type == BC_PRESSURE || B.type == BC_VELOCITY_PERIODIC) {
+                        double v = B.fp(t, x, y, z);
+                        if (what == BCD_PSOL) v = B.fp(t + s->dt, x, y, z) - v;
+                        h[3 * (size_t)n1 * n2 + o] = v;
+                    }
+                    if (norm2)
+                        for (int c = 0; c < 4; c++) norm2[c] += 2 * h[c * (size_t)n1 * n2 + o] * h[c * (size_t)n1 * n2 + o];
+                }
+            s->bc_arr[ax][hi].upload(h);
+            F.garr = s->bc_arr[ax][hi].p;
+        }
+    return op;
+}
+
+static void update_outer(IBMSolver *s) {
+    OuterParams ou = make_outer(s, BCD_U, s->time, nullptr);
+    TIMED("outer_u", 0, launch_outer(s->stream, s->geo, ou, s->u[0].p, s->u[1].p, s->u[2].p, 0, 0));
+    OuterParams opp = make_outer(s, BCD_P, s->time, nullptr);
+    TIMED("outer_p", 0, launch_outer(s->stream, s->geo, opp, s->p.p, nullptr, nullptr, 1, 0));
+}
+
+static void update_ghost(IBMSolver *s) {
+    TIMED("ghost_update", 0,
+          launch_ghost_update(s->stream, s->gc.p, s->ng, s->level_start.data(), (int)s->level_start.size() - 1,
+                              s->ghost_old.p, s->u[0].p, s->u[1].p, s->u[2].p, s->p.p));
+}
+
+static void read_sums(IBMSolver *s, int nv) {
+    HIP_CHECK(hipMemcpyAsync(s->sums_h, s->sums.p, nv * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+}
+
+static void read_ks(IBMSolver *s) {
+    HIP_CHECK(hipMemcpyAsync(s->ks_h, s->ks.p, sizeof(KScal), hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+}
+
+// ---------------------------------------------------------------------------
+// multigrid hierarchy of the pressure system
+// ---------------------------------------------------------------------------
+static void level_geometry(IBMSolver *s, MGLevel &L, const std::vector<double> &xf, const std::vector<double> &yf,
+                           const std::vector<double> &zf) {
+    const std::vector<double> *f[3] = {&xf, &yf, &zf};
+    std::vector<double> *D[3] = {&L.dx, &L.dy, &L.dz}, *C[3] = {&L.xc, &L.yc, &L.zc};
+    const int lo[3] = {3, 2, 4}, hi[3] = {1, 0, 5};
+    for (int a = 0; a < 3; a++) {
+        const std::vector<double> &F = *f[a];
+        const int N = (int)F.size() - 1;
+        std::vector<double> &dd = *D[a], &cc = *C[a];
+        dd.assign(N + 4, 0);
+        cc.assign(N + 4, 0);
+        for (int i = 0; i < N; i++) { H1(dd, i) = F[i + 1] - F[i]; H1(cc, i) = (F[i + 1] + F[i]) / 2; }
+        const bool plo = isperiodic(s->bc[lo[a]].type), phi = isperiodic(s->bc[hi[a]].type);
+        H1(dd, -1) = plo ? H1(dd, N - 1) : H1(dd, 0);
+        H1(dd, -2) = plo ? H1(dd, N - 2) : H1(dd, 1);
+        H1(dd, N) = phi ? H1(dd, 0) : H1(dd, N - 1);
+        H1(dd, N + 1) = phi ? H1(dd, 1) : H1(dd, N - 2);
+        H1(cc, -1) = H1(cc, 0) - (H1(dd, -1) + H1(dd, 0)) / 2;
+        H1(cc, -2) = H1(cc, -1) - (H1(dd, -2) + H1(dd, -1)) / 2;
+        H1(cc, N) = H1(cc, N - 1) + (H1(dd, N - 1) + H1(dd, N)) / 2;
+        H1(cc, N + 1) = H1(cc, N) + (H1(dd, N) + H1(dd, N + 1)) / 2;
+    }
+}
+
+static void kcoef(const IBMSolver *s, const std::vector<double> &D, const std::vector<double> &C, int N,
+                  std::vector<double> &km, std::vector<double> &kp) {
+    km.assign(N + 4, 0);
+    kp.assign(N + 4, 0);
+    for (int i = 0; i < N; i++) {
+        H1(km, i) = s->dt / (2 * s->Re * (H1(C, i) - H1(C, i - 1)) * H1(D, i));
+        H1(kp, i) = s->dt / (2 * s->Re * (H1(C, i + 1) - H1(C, i)) * H1(D, i));
+    }
+}
+
+void build_mg(IBMSolver *s) {
+    s->mg.clear();
+    s->mg.emplace_back();
+    {
+        MGLevel &L0 = s->mg[0];
+        L0.d = s->d;
+        L0.dx = s->dx; L0.dy = s->dy; L0.dz = s->dz;
+        L0.xc = s->xc; L0.yc = s->yc; L0.zc = s->zc;
+        L0.hflag = s->flag_dev_h;
+        std::vector<double> pk;
+        pk.insert(pk.end(), s->dx.begin(), s->dx.end());
+        pk.insert(pk.end(), s->dy.begin(), s->dy.end());
+        pk.insert(pk.end(), s->dz.begin(), s->dz.end());
+        L0.dxyz.upload(pk);
+    }
+    std::vector<double> xf = s->xf, yf = s->yf, zf = s->zf;
+    const int lo[3] = {3, 2, 4}, hi[3] = {1, 0, 5};
+    while (true) {
+        const MGLevel &P = s->mg.back();
+        const int n[3] = {P.d.nx, P.d.ny, P.d.nz};
+        int c[3];
+        for (int a = 0; a < 3; a++) c[a] = (n[a] % 2 == 0 && n[a] >= 4) ? 1 : 0;
+        if ((!c[0] && !c[1] && !c[2]) || (long)n[0] * n[1] * n[2] <= 512) break;
+        // coarse faces: every other face on coarsened axes
+        auto coarsen = [](const std::vector<double> &f, int cf) {
+            if (!cf) return f;
+            std::vector<double> g;
+            for (size_t q = 0; q < f.size(); q += 2) g.push_back(f[q]);
+            return g;
+        };
+        xf = coarsen(xf, c[0]); yf = coarsen(yf, c[1]); zf = coarsen(zf, c[2]);
+        MGLevel L;
+        L.cx = c[0]; L.cy = c[1]; L.cz = c[2];
+        const int m[3] = {(int)xf.size() - 1, (int)yf.size() - 1, (int)zf.size() - 1};
+        L.d = make_dims(m[0], m[1], m[2]);
+        level_geometry(s, L, xf, yf, zf);
+        // activity: any fluid child (outer layers active)
+        L.hflag.assign(L.d.ncell, F_FLUID);
+        for (int K = 0; K < m[2]; K++)
+            for (int J = 0; J < m[1]; J++)
+                for (int I = 0; I < m[0]; I++) {
+                    bool act = false;
+                    for (int kk = 0; kk <= c[2]; kk++)
+                        for (int jj = 0; jj <= c[1]; jj++)
+                            for (int ii = 0; ii <= c[0]; ii++) {
+                                const int i = c[0] ? 2 * I + ii : I, j = c[1] ? 2 * J + jj : J, k = c[2] ? 2 * K + kk : K;
+                                if (P.hflag[cidx(P.d, i, j, k)] == F_FLUID) act = true;
+                            }
+                    L.hflag[cidx(L.d, I, J, K)] = act ? F_FLUID : F_SOLID;
+                }
+        L.flag.upload(L.hflag);
+        std::vector<double> pk;
+        pk.insert(pk.end(), L.dx.begin(), L.dx.end());
+        pk.insert(pk.end(), L.dy.begin(), L.dy.end());
+        pk.insert(pk.end(), L.dz.begin(), L.dz.end());
+        L.dxyz.upload(pk);
+        // rediscretised operator with the same boundary folds
+        std::vector<double> kW, kE, kS, kN, kD, kU;
+        kcoef(s, L.dx, L.xc, m[0], kW, kE);
+        kcoef(s, L.dy, L.yc, m[1], kS, kN);
+        kcoef(s, L.dz, L.zc, m[2], kD, kU);
+        const std::vector<double> *D[3] = {&L.dx, &L.dy, &L.dz}, *C[3] = {&L.xc, &L.yc, &L.zc};
+        const std::vector<double> *Km[3] = {&kW, &kS, &kD}, *Kp[3] = {&kE, &kN, &kU};
+        IBMSolverBCType bct[3][2];
+        for (int a = 0; a < 3; a++)
+            for (int h = 0; h < 2; h++) bct[a][h] = s->bc[face_bc(a, h)].type;
+        build_system(s, D, C, Km, Kp, m, bct, 3, L.sys);
+        L.sys.S64.skip = L.sys.S32.skip = 1;
+        // trilinear prolongation data over the finer grid's indices
+        const std::vector<double> *cf[3] = {&P.xc, &P.yc, &P.zc}, *cc[3] = {&L.xc, &L.yc, &L.zc};
+        DBuf<int> *pidx[3] = {&L.pi, &L.pj, &L.pk};
+        DBuf<double> *pw[3] = {&L.wi, &L.wj, &L.wk};
+        for (int a = 0; a < 3; a++) {
+            std::vector<int> far(n[a]);
+            std::vector<double> w(n[a]);
+            const bool per = s->bc[lo[a]].type == BC_ALL_PERIODIC && s->bc[hi[a]].type == BC_ALL_PERIODIC;
+            for (int i = 0; i < n[a]; i++) {
+                if (!c[a]) { far[i] = i; w[i] = 1; continue; }
+                const int I = i >> 1;
+                int J = (i & 1) ? I + 1 : I - 1;
+                const double xf_ = H1((*cf[a]), i), Xn = H1((*cc[a]), I);
+                if (J < 0 || J >= m[a]) {
+                    if (per) {
+                        const double Xf = H1((*cc[a]), J);  // ghost centre of the periodic image
+                        w[i] = (Xf - xf_) / (Xf - Xn);
+                        far[i] = (J + m[a]) % m[a];
+                    } else {
+                        const IBMSolverBCType t = s->bc[J < 0 ? lo[a] : hi[a]].type;
+                        const bool dir = t == BC_PRESSURE || t == BC_VELOCITY_PERIODIC;
+                        const double Xf = H1((*cc[a]), J);
+                        const double wn = (Xf - xf_) / (Xf - Xn);
+                        w[i] = dir ? wn - (1 - wn) : 1.0;  // Dirichlet: odd mirror, Neumann: even
+                        far[i] = I;
+                    }
+                } else {
+                    const double Xf = H1((*cc[a]), J);
+                    w[i] = (Xf - xf_) / (Xf - Xn);
+                    far[i] = J;
+                }
+            }
+            pidx[a]->upload(far);
+            pw[a]->upload(w);
+        }
+        L.tr = {c[0], c[1], c[2], L.pi.p, L.pj.p, L.pk.p, L.wi.p, L.wj.p, L.wk.p, nullptr};
+        L.x64.alloc(L.d.ncell); L.b64.alloc(L.d.ncell); L.r64.alloc(L.d.ncell);
+        if (s->precision == IBM3D_PREC_MIXED) { L.x32.alloc(L.d.ncell); L.b32.alloc(L.d.ncell); L.r32.alloc(L.d.ncell); }
+        s->mg.push_back(std::move(L));
+    }
+    // fine-level residual scratch
+    s->mg[0].r64.alloc(s->d.ncell);
+    if (s->precision == IBM3D_PREC_MIXED) s->mg[0].r32.alloc(s->d.ncell);
+}
+
+template <typename T> static DBuf<T> &lev_r(MGLevel &L);
+template <> DBuf<double> &lev_r<double>(MGLevel &L) { return L.r64; }
+template <> DBuf<float> &lev_r<float>(MGLevel &L) { return L.r32; }
+template <typename T> static DBuf<T> &lev_x(MGLevel &L);
+template <> DBuf<double> &lev_x<double>(MGLevel &L) { return L.x64; }
+template <> DBuf<float> &lev_x<float>(MGLevel &L) { return L.x32; }
+template <typename T> static DBuf<T> &lev_b(MGLevel &L);
+template <> DBuf<double> &lev_b<double>(MGLevel &L) { return L.b64; }
+template <> DBuf<float> &lev_b<float>(MGLevel &L) { return L.b32; }
+template <typename T> static const SysCoef<T> &sysc(const SysData &S);
+template <> const SysCoef<double> &sysc<double>(const SysData &S) { return S.S64; }
+template <> const SysCoef<float> &sysc<float>(const SysData &S) { return S.S32; }
+template <typename T> static const GhostRows<T> &grows(const GhostData &G);
+template <> const GhostRows<double> &grows<double>(const GhostData &G) { return G.G64; }
+template <> const GhostRows<float> &grows<float>(const GhostData &G) { return G.G32; }
+template <typename T> static DBuf<T> *kvec(Krylov &K);
+template <> DBuf<double> *kvec<double>(Krylov &K) { return K.v64; }
+template <> DBuf<float> *kvec<float>(Krylov &K) { return K.v32; }
+
+static const int kPre = 2, kPost = 2, kCoarse = 100;
+
+// x = V-cycle approximation of A^-1 b on level l (x0 = 0)
+template <typename T>
+static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
+    MGLevel &L = s->mg[l];
+    const bool fine = l == 0;
+    const SysCoef<T> &S = fine ? sysc<T>(s->sys[3]) : sysc<T>(L.sys);
+    const GhostRows<T> G0 = fine ? grows<T>(s->gp) : GhostRows<T>{0, nullptr, nullptr, nullptr, nullptr};
+    const uint8_t *flag = fine ? s->flag.p : L.flag.p;
+    const double cell_b = (double)L.d.nx * L.d.ny * L.d.nz * sizeof(T);
+    if (l + 1 == (int)s->mg.size()) {
+        TIMED("mg_coarse", 0, launch_coarse_solve<T>(s->stream, L.d, S, flag, b, x, kCoarse, s->p_singular ? 1 : 0,
+                                                     L.dxyz.p));
+        return;
+    }
+    TIMED("mg_zero", cell_b, launch_zero<T>(s->stream, L.d, x));
+    TIMED(fine ? "mg_smooth_fine" : "mg_smooth", 2 * kPre * 3 * cell_b / 2,
+          launch_rbgs<T>(s->stream, L.d, S, G0, flag, b, x, kPre));
+    NormSpec ns{0, nullptr, nullptr, nullptr, nullptr, -1};
+    TIMED(fine ? "mg_resid_fine" : "mg_resid", 3 * cell_b,
+          launch_residual<T>(s->stream, L.d, S, G0, flag, b, x, lev_r<T>(L).p, ns, s->partial.p));
+    MGLevel &Cl = s->mg[l + 1];
+    TIMED("mg_restrict", cell_b * 1.125,
+          launch_restrict<T>(s->stream, L.d, Cl.d, Cl.tr, flag, lev_r<T>(L).p, L.dxyz.p + 2,
+                             L.dxyz.p + (L.d.nx + 4) + 2, L.dxyz.p + (L.d.nx + 4) + (L.d.ny + 4) + 2,
+                             Cl.dxyz.p + 2, Cl.dxyz.p + (Cl.d.nx + 4) + 2, Cl.dxyz.p + (Cl.d.nx + 4) + (Cl.d.ny + 4) + 2,
+                             lev_b<T>(Cl).p));
+    vcycle<T>(s, l + 1, lev_b<T>(Cl).p, lev_x<T>(Cl).p);
+    TIMED("mg_prolong", cell_b * 2.125,
+          launch_prolong<T>(s->stream, L.d, Cl.d, Cl.tr, flag, Cl.flag.p, lev_x<T>(Cl).p, x));
+    TIMED(fine ? "mg_smooth_fine" : "mg_smooth", 2 * kPost * 3 * cell_b / 2,
+          launch_rbgs<T>(s->stream, L.d, S, G0, flag, b, x, kPost));
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB (van der Vorst 1992), right-preconditioned, device scalars.
+// prec: 0 none, 1 Jacobi, 2 multigrid (pressure only).
+// bnorm > 0: stopping norm of the reference system's right-hand side;
+// bnorm <= 0: use ||b|| of this system (inner solves).
+// ---------------------------------------------------------------------------
+template <typename T>
+static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, const T *b, T *x, int prec,
+                    const NormSpec &ns, double tol, int maxit, double bnorm, double *final_rel, const char *tag) {
+    const Dims &d = s->d;
+    DBuf<T> *V = kvec<T>(s->kry);
+    T *r = V[0].p, *rh = V[1].p, *p = V[2].p, *v = V[3].p, *ph = V[4].p, *sv = V[5].p, *sh = V[6].p, *t = V[7].p;
+    const uint8_t *flag = s->flag.p;
+    const long nv = vec_partials(d), na = apply_partials(d, G.ng);
+    const double cb = ncells(s) * sizeof(T);
+    char nm[64];
+
+    launch_bicg_init<T>(s->stream, d, r, rh, p, v, x, b);
+    // norms of b (= r0) through the residual kernel with x = 0
+    TIMED("residual", 3 * cb, launch_residual<T>(s->stream, d, S, G, flag, b, x, r, ns, s->partial.p));
+    launch_finalize(s->stream, 3, s->partial.p, na, 2, s->ks.p);
+    read_ks(s);
+    const double bn = bnorm > 0 ? bnorm : std::sqrt(ns.on ? s->ks_h->bbn : s->ks_h->bb);
+    if (bn == 0) { *final_rel = 0; return 0; }
+    double rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
+    auto precond = [&](const T *in, T *out) {
+        if (prec == 1) {
+            TIMED("jacobi", 2 * cb, launch_jacobi<T>(s->stream, d, S, G, flag, in, out));
+        } else if (prec == 2) {
+            vcycle<T>(s, 0, in, out);
+        } else {
+            launch_copy<T>(s->stream, d, in, out);
+        }
+    };
+    int it = 0;
+    while (it < maxit && rn > tol * bn) {
+        it++;
+        snprintf(nm, sizeof nm, "bicg_vec");
+        TIMED("bicg_vec", 4 * cb, launch_bicg_p<T>(s->stream, d, s->ks.p, r, v, p));
+        precond(p, ph);
+        snprintf(nm, sizeof nm, "spmv_%s", tag);
+        TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, ph, v, 1, rh, s->partial.p));
+        launch_finalize(s->stream, 0, s->partial.p, na, 1, s->ks.p);
+        TIMED("bicg_vec", 3 * cb, launch_bicg_s<T>(s->stream, d, s->ks.p, r, v, sv));
+        precond(sv, sh);
+        TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, sh, t, 2, sv, s->partial.p));
+        launch_finalize(s->stream, 1, s->partial.p, na, 2, s->ks.p);
+        TIMED("bicg_xr", 8 * cb, launch_bicg_xr<T>(s->stream, d, s->ks.p, ph, sh, sv, t, rh, x, r, ns, s->partial.p));
+        launch_finalize(s->stream, 2, s->partial.p, nv, 3, s->ks.p);
+        read_ks(s);
+        rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
+        if (s->ks_h->breakdown || !std::isfinite(rn)) break;
+    }
+    // true residual of the final iterate
+    TIMED("residual", 3 * cb, launch_residual<T>(s->stream, d, S, G, flag, b, x, r, ns, s->partial.p));
+    launch_finalize(s->stream, 4, s->partial.p, na, 2, s->ks.p);
+    read_ks(s);
+    *final_rel = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr) / bn;
+    return it;
+}
+
+// FP64 iterative refinement around FP32 BiCGSTAB inner solves.
+static int refine_mixed(IBMSolver *s, int sk, const double *b, double *x, int prec, const NormSpec &ns, double tol,
+                        int maxit, double bnorm, double *final_rel) {
+    const Dims &d = s->d;
+    const SysCoef<double> &S64 = s->sys[sk].S64;
+    const SysCoef<float> &S32 = s->sys[sk].S32;
+    const GhostData &GD = sk == 3 ? s->gp : s->gmom;
+    const uint8_t *flag = s->flag.p;
+    const long na = apply_partials(d, GD.G64.ng);
+    double *r64 = s->ref_r64.p;
+    float *rhs32 = s->ref_rhs32.p, *e32 = s->ref_e32.p;
+    const double cb = ncells(s);
+    launch_zero<double>(s->stream, d, x);
+    TIMED("residual64", 3 * cb * 8, launch_residual<double>(s->stream, d, S64, GD.G64, flag, b, x, r64, ns, s->partial.p));
+    launch_finalize(s->stream, 4, s->partial.p, na, 2, s->ks.p);
+    read_ks(s);
+    const double bn = bnorm > 0 ? bnorm : std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
+    if (bn == 0) { *final_rel = 0; return 0; }
+    double rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
+    int total = 0, outer = 0;
+    const NormSpec off{0, nullptr, nullptr, nullptr, nullptr, -1};
+    const double inner_tol = 1e-4;
+    while (rn > tol * bn && outer < 20 && total < maxit) {
+        outer++;
+        TIMED("d2f", cb * 12, launch_d2f(s->stream, d, r64, rhs32));
+        double rel = 0;
+        const int it = bicgstab<float>(s, S32, GD.G32, rhs32, e32, prec, off, inner_tol, 200, -1.0, &rel,
+                                       sk == 3 ? "p32" : "u32");
+        total += std::max(it, 1);
+        TIMED("add_f2d", cb * 20, launch_add_f2d(s->stream, d, e32, x));
+        TIMED("residual64", 3 * cb * 8,
+              launch_residual<double>(s->stream, d, S64, GD.G64, flag, b, x, r64, ns, s->partial.p));
+        launch_finalize(s->stream, 4, s->partial.p, na, 2, s->ks.p);
+        read_ks(s);
+        const double rn_new = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
+        if (!std::isfinite(rn_new)) { rn = rn_new; break; }
+        rn = rn_new;
+    }
+    *final_rel = rn / bn;
+    return total;
+}
+
+static void solve_system(IBMSolver *s, int sk, const double *b, double *x, double bnorm) {
+    const int prec = sk == 3 ? (s->precond_type == PRECOND_AMG ? 2 : 1) : 1;
+    NormSpec ns{0, nullptr, nullptr, nullptr, nullptr, -1};
+    if (sk == 3) ns = {1, s->ks1d[0].p, s->ks1d[1].p, s->ks1d[2].p, s->flag.p, s->pin};
+    const int maxit = sk == 3 ? 1000 : 200;  // src/ibm3d_setup.c:1413, 1467
+    double rel = 0;
+    int it;
+    if (s->precision == IBM3D_PREC_MIXED) {
+        it = refine_mixed(s, sk, b, x, prec, ns, s->tol, maxit, bnorm, &rel);
+    } else {
+        const GhostData &GD = sk == 3 ? s->gp : s->gmom;
+        it = bicgstab<double>(s, s->sys[sk].S64, GD.G64, b, x, prec, ns, s->tol, maxit, bnorm, &rel,
+                              sk == 3 ? "p64" : "u64");
+    }
+    s->final_norm[sk] = rel;
+    s->final_iter[sk] = it;
+    if (!std::isfinite(rel)) {
+        if (sk < 3) fprintf(stderr, "error: floating pointer error raised in u%d_star\n", sk + 1);
+        else fprintf(stderr, "error: floating pointer error raised in p_prime\n");
+        abort();  // src/ibm3d_fracstep.c:363-366, 673-676
+    }
+    if (rel > s->tol) {
+        if (sk < 3) fprintf(stderr, "warning: u%d_star did not converge\n", sk + 1);
+        else fprintf(stderr, "warning: p_prime did not converge\n");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one fractional step
+// ---------------------------------------------------------------------------
+static StepParams step_params(IBMSolver *s) {
+    StepParams sp{};
+    sp.dt = s->dt;
+    sp.Re = s->Re;
+    for (int l = 0; l < 3; l++) { sp.force[l] = s->const_f[l]; sp.fa[l] = nullptr; }
+    if (s->force_type == VAL_FUNC) {
+        // host-evaluated force over the interior cells (src/ibm3d_fracstep.c:266)
+        for (int l = 0; l < 3; l++) {
+            std::vector<double> h(s->d.ncell, 0.0);
+            for (int k = 0; k < s->Nz; k++)
+                for (int j = 0; j < s->Ny; j++)
+                    for (int i = 0; i < s->Nx; i++)
+                        h[cidx(s->d, i, j, k)] = s->func_f[l](s->time, H1(s->xc, i), H1(s->yc, j), H1(s->zc, k));
+            s->force_arr[l].upload(h);
+            sp.fa[l] = s->force_arr[l].p;
+        }
+    }
+    return sp;
+}
+
+static TauPtrs tau_ptrs(IBMSolver *s) {
+    TauPtrs tp{};
+    tp.on = s->turb_type != TURBMODEL_NONE;
+    static const int slot[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    for (int a = 0; a < 6; a++) tp.t[a] = s->tau[a].p;
+    for (int l = 0; l < 3; l++)
+        for (int m = 0; m < 3; m++) tp.slot[l][m] = slot[l][m];
+    return tp;
+}
+
+static void calc_tau(IBMSolver *s) {
+    if (s->turb_type == TURBMODEL_NONE) return;
+    const double *u3[3] = {s->u[0].p, s->u[1].p, s->u[2].p};
+    TIMED("sgs", ncells(s) * (3 + 6) * 8 + ncells(s),
+          launch_sgs(s->stream, s->geo, s->turb_type == TURBMODEL_SMAGORINSKY ? 1 : 2, s->turb_coef, s->flag.p,
+                     s->lvset.p, u3, tau_ptrs(s)));
+}
+
+static void do_step(IBMSolver *s) {
+    const StepParams sp = step_params(s);
+    const double cn = ncells(s);
+    calc_tau(s);
+    // K1: convection + momentum RHS
+    {
+        const double *u3[3] = {s->u[0].p, s->u[1].p, s->u[2].p};
+        const double *U3[3] = {s->F[0].p, s->F[1].p, s->F[2].p};
+        const double *Np[3] = {s->Np[0].p, s->Np[1].p, s->Np[2].p};
+        double *N[3] = {s->N[0].p, s->N[1].p, s->N[2].p};
+        double *b[3] = {s->b[0].p, s->b[1].p, s->b[2].p};
+        TIMED("k1_conv_rhs", cn * (16 * 8 + 1),
+              launch_conv_rhs(s->stream, s->geo, sp, s->flag.p, u3, U3, s->p.p, Np, N, b, tau_ptrs(s),
+                              s->partial.p));
+        launch_sum_partials(s->stream, s->partial.p, conv_rhs_blocks(s->d), 3, s->sums.p, 0);
+    }
+    double fnorm[4];
+    const OuterParams ob = make_outer(s, BCD_USOL, s->time, fnorm);
+    for (int l = 0; l < 3; l++)
+        for (int ax = 0; ax < 3; ax++)
+            for (int hi = 0; hi < 2; hi++) {
+                const double gm = s->sys[l].gamma[ax][hi];
+                if (gm == 0 || s->bc[face_bc(ax, hi)].type != BC_VELOCITY_COMPONENT) continue;
+                const double *Kf = ax == 0 ? (hi ? s->g1d[7].p : s->g1d[6].p)
+                                 : ax == 1 ? (hi ? s->g1d[9].p : s->g1d[8].p) : (hi ? s->g1d[11].p : s->g1d[10].p);
+                launch_bc_rhs(s->stream, s->geo, ob, ax, hi, l, Kf, gm, s->b[l].p);
+            }
+    read_sums(s, 3);
+    double bn2[3];
+    for (int l = 0; l < 3; l++) bn2[l] = s->sums_h[l] + s->sys[l].bc_norm2 + fnorm[l];
+    // momentum Helmholtz solves
+    for (int l = 0; l < 3; l++) {
+        solve_system(s, l, s->b[l].p, s->us[l].p, std::sqrt(bn2[l]));
+        TIMED("outer_sol", 0, launch_outer(s->stream, s->geo, ob, s->us[l].p, nullptr, nullptr, 2, l));
+    }
+    // K2: U*, Poisson RHS
+    {
+        const double *us3[3] = {s->us[0].p, s->us[1].p, s->us[2].p};
+        double *Fs[3] = {s->Fs[0].p, s->Fs[1].p, s->Fs[2].p};
+        TIMED("k2_face_star", cn * (4 * 8 + 3 * 8 + 1), launch_face_star(s->stream, s->geo, sp, s->flag.p, us3, s->p.p, Fs));
+        const double *Fc[3] = {s->Fs[0].p, s->Fs[1].p, s->Fs[2].p};
+        TIMED("k2_poisson_rhs", cn * (4 * 8 + 1),
+              launch_poisson_rhs(s->stream, s->geo, sp, s->flag.p, s->pin, Fc, s->bp.p, s->partial.p));
+        launch_sum_partials(s->stream, s->partial.p, conv_rhs_blocks(s->d), 1, s->sums.p, 0);
+    }
+    double pnorm[4];
+    const OuterParams opp = make_outer(s, BCD_PSOL, s->time, pnorm);
+    for (int ax = 0; ax < 3; ax++)
+        for (int hi = 0; hi < 2; hi++) {
+            const IBMSolverBCType t = s->bc[face_bc(ax, hi)].type;
+            if ((t != BC_PRESSURE && t != BC_VELOCITY_PERIODIC) || s->bc[face_bc(ax, hi)].val_type != VAL_FUNC) continue;
+            const double *Kf = ax == 0 ? (hi ? s->g1d[7].p : s->g1d[6].p)
+                             : ax == 1 ? (hi ? s->g1d[9].p : s->g1d[8].p) : (hi ? s->g1d[11].p : s->g1d[10].p);
+            launch_bc_rhs(s->stream, s->geo, opp, ax, hi, 3, Kf, s->sys[3].gamma[ax][hi], s->bp.p);
+        }
+    read_sums(s, 1);
+    const double bnp = std::sqrt(s->sums_h[0] + s->sys[3].bc_norm2 + pnorm[3]);
+    solve_system(s, 3, s->bp.p, s->pp.p, bnp);
+    TIMED("outer_sol", 0, launch_outer(s->stream, s->geo, opp, s->pp.p, nullptr, nullptr, 3, 0));
+    // K3: projection
+    {
+        const double *us3[3] = {s->us[0].p, s->us[1].p, s->us[2].p};
+        const double *Fs[3] = {s->Fs[0].p, s->Fs[1].p, s->Fs[2].p};
+        double *un[3] = {s->un[0].p, s->un[1].p, s->un[2].p};
+        double *Fn[3] = {s->Fn[0].p, s->Fn[1].p, s->Fn[2].p};
+        TIMED("k3_update", cn * (15 * 8 + 1),
+              launch_update(s->stream, s->geo, sp, s->flag.p, s->p.p, s->pp.p, us3, Fs, s->pn.p, un, Fn));
+    }
+    // pointer swaps  <- src/ibm3d_fracstep.c:795-804
+    for (int l = 0; l < 3; l++) {
+        std::swap(s->u[l], s->un[l]);
+        std::swap(s->F[l], s->Fn[l]);
+        std::swap(s->Np[l], s->N[l]);
+    }
+    std::swap(s->p, s->pn);
+}
+
+void iterate_impl(IBMSolver *s, int steps, bool verbose) {
+    const double t0 = s->time;
+    // prologue  <- src/ibm3d_fracstep.c:46-54
+    update_outer(s);
+    launch_interp_stag(s->stream, s->geo, s->u[0].p, s->u[1].p, s->u[2].p, s->F[0].p, s->F[1].p, s->F[2].p);
+    launch_solid_nan(s->stream, s->d, s->flag.p, s->u[0].p, s->u[1].p, s->u[2].p, s->p.p);
+    update_ghost(s);
+    {
+        const StepParams sp = step_params(s);
+        const double *u3[3] = {s->u[0].p, s->u[1].p, s->u[2].p};
+        const double *U3[3] = {s->F[0].p, s->F[1].p, s->F[2].p};
+        const double *Np[3] = {s->Np[0].p, s->Np[1].p, s->Np[2].p};
+        double *N[3] = {s->N[0].p, s->N[1].p, s->N[2].p};
+        double *b[3] = {s->b[0].p, s->b[1].p, s->b[2].p};
+        TauPtrs tp{};
+        tp.on = 0;
+        launch_conv_rhs(s->stream, s->geo, sp, s->flag.p, u3, U3, s->p.p, Np, N, b, tp, s->partial.p);
+        for (int l = 0; l < 3; l++) std::swap(s->N[l], s->Np[l]);
+    }
+    for (int i = 0; i < steps;) {
+        do_step(s);
+        i++;
+        s->iter++;
+        s->time = t0 + s->dt * i;
+        update_outer(s);
+        update_ghost(s);
+        if (verbose) {
+            HIP_CHECK(hipStreamSynchronize(s->stream));
+            if (s->iter % 10 == 1 || i == 1) {
+                printf("\n  iter       u1 res       u2 res       u3 res        p res   its(u1 u2 u3 p)\n");
+                printf("-------------------------------------------------------------------------------\n");
+            }
+            printf("%6d   %10.4e   %10.4e   %10.4e   %10.4e   %d %d %d %d\n", s->iter, s->final_norm[0],
+                   s->final_norm[1], s->final_norm[2], s->final_norm[3], s->final_iter[0], s->final_iter[1],
+                   s->final_iter[2], s->final_iter[3]);
+            fflush(stdout);
+        }
+        if (s->autosave_period > 0 && s->iter % s->autosave_period == 0 && s->autosave_filename) {
+            char name[256];
+            snprintf(name, sizeof name, "%s-%05d", s->autosave_filename, s->iter);  // :847-853
+            export_result_impl(s, name);
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+}
+
+double max_div_impl(IBMSolver *s) {
+    const double *U3[3] = {s->F[0].p, s->F[1].p, s->F[2].p};
+    launch_maxdiv(s->stream, s->geo, s->flag.p, U3, s->partial.p);
+    launch_sum_partials(s->stream, s->partial.p, maxdiv_blocks(s->d), 1, s->sums.p, 1);
+    read_sums(s, 1);
+    return s->sums_h[0];
+}
+
+}  // namespace ibm
+
+using namespace ibm;
+
+extern "C" {
+
+void IBMSolver_enable_timing(IBMSolver *s, int on) { s->timing = on != 0; }
+
+void IBMSolver_reset_timing(IBMSolver *s) {
+    timers_flush(s);
+    s->timers.clear();
+}
+
+int IBMSolver_get_kernel_stat(IBMSolver *s, int idx, char *name, int namelen, double *total_ms, long *count,
+                              double *bytes) {
+    timers_flush(s);
+    if (idx >= 0 && idx < (int)s->timers.size()) {
+        const KTimer &t = s->timers[idx];
+        if (name && namelen > 0) { strncpy(name, t.name.c_str(), namelen - 1); name[namelen - 1] = 0; }
+        if (total_ms) *total_ms = t.total_ms;
+        if (count) *count = t.count;
+        if (bytes) *bytes = t.bytes;
+    }
+    return (int)s->timers.size();
+}
+
+void IBMSolver_calc_tau_r(IBMSolver *s) { calc_tau(s); }
+
+}  // extern "C"

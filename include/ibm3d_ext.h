@@ -1,0 +1,87 @@
+/*
+ * ibm3d_ext.h - extensions of the solver API. None of these has a counterpart
+ * in the reference; they exist for callers that need what the reference gets
+ * by reading the public IBMSolver struct (src/ibm3d.h:100-213: fields, flags,
+ * residual norms) and for the GPU-only controls (precision mode, timing).
+ *
+ * Array layouts returned by IBMSolver_get_field are the reference's host
+ * layouts (src/utils.h:8-16), so a caller can compare against the reference
+ * struct fields element by element:
+ *   cell fields   (Nx+4)(Ny+4)(Nz+4), index ((i+2)*(Ny+4) + j+2)*(Nz+4) + k+2
+ *   x-face field  (Nx+1)(Ny+4)(Nz+4), index (i*(Ny+4) + j+2)*(Nz+4) + k+2
+ *   y-face field  (Nx+4)(Ny+1)(Nz+4), index ((i+2)*(Ny+1) + j)*(Nz+4) + k+2
+ *   z-face field  (Nx+4)(Ny+4)(Nz+1), index ((i+2)*(Ny+4) + j+2)*(Nz+1) + k
+ */
+#ifndef IBM3D_EXT_H
+#define IBM3D_EXT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum _ibm3d_field {
+    IBM3D_FIELD_U1 = 0, IBM3D_FIELD_U2, IBM3D_FIELD_U3, IBM3D_FIELD_P,
+    IBM3D_FIELD_U1_STAR, IBM3D_FIELD_U2_STAR, IBM3D_FIELD_U3_STAR,
+    IBM3D_FIELD_P_PRIME,
+    IBM3D_FIELD_N1, IBM3D_FIELD_N2, IBM3D_FIELD_N3,
+    IBM3D_FIELD_LVSET,
+    IBM3D_FIELD_FACE_U1, IBM3D_FIELD_FACE_U2, IBM3D_FIELD_FACE_U3,
+    IBM3D_FIELD_TAU11, IBM3D_FIELD_TAU12, IBM3D_FIELD_TAU13,
+    IBM3D_FIELD_TAU22, IBM3D_FIELD_TAU23, IBM3D_FIELD_TAU33,
+    IBM3D_FIELD_COUNT
+} IBM3DField;
+
+typedef enum _ibm3d_precision {
+    IBM3D_PREC_FP64 = 0,   /* every array and solve in FP64 */
+    IBM3D_PREC_MIXED = 1,  /* FP32 smoother / multigrid inside FP64 refinement */
+} IBM3DPrecision;
+
+/* Analytic obstacle phi = |x - c| - r (positive in the fluid). Replaces
+   IBMSolver_set_obstacle's STL level set for the sphere case. */
+void IBMSolver_set_obstacle_sphere(IBMSolver *, double cx, double cy, double cz,
+                                   double r);
+
+/* Precision mode of the linear solves (default IBM3D_PREC_FP64). Must be
+   called before IBMSolver_assemble. */
+void IBMSolver_set_precision(IBMSolver *, int mode);
+
+/* Copies a field to host memory in the reference layout (see above). Solid
+   cells hold NaN where the reference stores NaN. */
+void IBMSolver_get_field(IBMSolver *, int field, double *out);
+/* Overwrites u1, u2, u3 or p (cell fields only, reference layout). */
+void IBMSolver_set_field(IBMSolver *, int field, const double *in);
+/* Cell flags FLAG_FLUID/GHOST/SOLID, cell layout. */
+void IBMSolver_get_flag(IBMSolver *, int *out);
+
+int IBMSolver_get_iter(IBMSolver *);
+double IBMSolver_get_time(IBMSolver *);
+
+/* Final relative residual norms (u1, u2, u3, p) and iteration counts of the
+   last time step, as printed by the reference's verbose table. */
+void IBMSolver_get_last_norms(IBMSolver *, double out[4]);
+void IBMSolver_get_last_iters(IBMSolver *, int out[4]);
+
+/* Interior ghost cells in the reference's lexicographic order (i, j, k; k
+   fastest): writes 3*count ints when out != NULL, returns count. */
+int IBMSolver_get_ghost_cells(IBMSolver *, int *out);
+
+/* max |div U| over fluid cells, U the face velocity after projection. */
+double IBMSolver_max_div(IBMSolver *);
+
+/* Pressure + viscous force on the immersed body from the ghost-cell surface
+   (Fx, Fy, Fz); drag and lift coefficients follow by scaling. */
+void IBMSolver_calc_body_force(IBMSolver *, double out[3]);
+
+/* Per-kernel timing with HIP events on the solver's stream. */
+void IBMSolver_enable_timing(IBMSolver *, int on);
+void IBMSolver_reset_timing(IBMSolver *);
+/* Returns the number of timed kernels; fills entry idx when idx is valid:
+   name, accumulated milliseconds, launch count, algorithmic bytes moved. */
+int IBMSolver_get_kernel_stat(IBMSolver *, int idx, char *name, int namelen,
+                              double *total_ms, long *count, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

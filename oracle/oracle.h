@@ -1,0 +1,113 @@
+/*
+ * oracle.h - internal header of the FP64 CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY. The oracle is a clean-room CPU restatement of the
+ * reference's fractional-step IBM solver (thecasterian/2020ResearchProject,
+ * src/ibm3d_*.c, src/utils.c, src/geo3d.c) used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker.
+ * The product (2020researchproject_amd/) never links or calls it.
+ *
+ * Layout is the reference's (src/utils.h:8-16): padded cell arrays with two
+ * ghost layers, k fastest; x/y/z face arrays (Nx+1)(Ny+4)(Nz+4) etc. One
+ * process holds the whole domain (the reference's Px = Py = Pz = 1 branch), so
+ * the global row number of cell (i,j,k) equals its padded linear offset
+ * (src/ibm3d_setup.c:751-757).
+ */
+#ifndef IBM3D_ORACLE_H
+#define IBM3D_ORACLE_H
+
+#include <math.h>
+#include <stdbool.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/ibm3d.h"
+
+#define C1(a, i) ((a)[(i) + 2])
+#define CI(s, i, j, k) ((((i) + 2) * ((s)->Ny + 4) + (j) + 2) * ((s)->Nz + 4) + (k) + 2)
+#define C3(s, a, i, j, k) ((a)[CI(s, i, j, k)])
+#define XS(s, a, i, j, k) ((a)[((i) * ((s)->Ny + 4) + (j) + 2) * ((s)->Nz + 4) + (k) + 2])
+#define YS(s, a, i, j, k) ((a)[(((i) + 2) * ((s)->Ny + 1) + (j)) * ((s)->Nz + 4) + (k) + 2])
+#define ZS(s, a, i, j, k) ((a)[(((i) + 2) * ((s)->Ny + 4) + (j) + 2) * ((s)->Nz + 1) + (k)])
+
+/* Sparse matrix with at most 9 entries per row (ELL), rows = padded cells. */
+typedef struct {
+    int nrows;
+    int *nnz;     /* nrows */
+    int *col;     /* 9 * nrows */
+    double *val;  /* 9 * nrows */
+    /* ILU(0) factors (same pattern, rows sorted by column), built lazily. */
+    int *scol;
+    double *lu;
+    int *diagpos;
+    bool has_ilu;
+} EllMat;
+
+typedef struct {
+    IBMSolverBCType type;
+    IBMSolverValType val_type;
+    double const_u1, const_u2, const_u3, const_p;
+    IBMSolverBCValFunc func_u1, func_u2, func_u3, func_p;
+} OBC;
+
+struct _ibm_solver {
+    int Nx, Ny, Nz;
+    double Re, dt;
+    int iter;
+    double time;
+    const char *autosave_filename;
+    int autosave_period;
+
+    double *dx, *dy, *dz, *xc, *yc, *zc;       /* c1e arrays, N+4 */
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+    double *kx_W, *kx_E, *ky_S, *ky_N, *kz_D, *kz_U;
+
+    OBC bc[6];
+
+    IBMSolverValType force_type;
+    double const_f[3];
+    IBMSolverForceFunc func_f[3];
+
+    IBMSolverTurbModelType turb_type;
+    double Cs;
+
+    Polyhedron *poly;
+    int has_sphere;
+    double sphere[4];
+
+    int *flag;
+    double *lvset;
+
+    double *u1, *u1_next, *u1_star, *u1_tilde;
+    double *u2, *u2_next, *u2_star, *u2_tilde;
+    double *u3, *u3_next, *u3_star, *u3_tilde;
+    double *U1, *U1_next, *U1_star;
+    double *U2, *U2_next, *U2_star;
+    double *U3, *U3_next, *U3_star;
+    double *p, *p_next, *p_prime;
+    double *N1, *N1_prev, *N2, *N2_prev, *N3, *N3_prev;
+    double *S[6], *tau[6];   /* 11,12,13,22,23,33 */
+    double *p_coeffsum;
+    int *cell_idx_periodic;
+
+    EllMat A[4];  /* u1, u2, u3, p */
+
+    IBMSolverLinearSolverType solver_type;
+    IBMSolverPrecondType precond_type;
+    double tol;
+
+    double *b, *x, *work;  /* solver vectors, P each; work = 8 P */
+    double final_norm[4];
+    int final_iter[4];
+};
+
+/* geo3d_oracle.c */
+double oracle_lvset_sphere(const double *c, double x, double y, double z);
+
+/* solver helpers in ibm3d_oracle.c */
+int oracle_bicgstab(const EllMat *A, const double *b, double *x, double tol,
+                    int maxit, int use_ilu, double *work, double *final_norm);
+void oracle_ilu0_build(EllMat *A);
+
+#endif
