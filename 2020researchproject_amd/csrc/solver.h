@@ -174,6 +174,7 @@ struct _ibm_solver {
     double final_norm[4] = {0, 0, 0, 0};
     int final_iter[4] = {0, 0, 0, 0};
     bool timing = false;
+    std::vector<double> step_ms;          // wall time of each step of the last iterate call
     std::vector<ibm::KTimer> timers;
     std::vector<hipEvent_t> event_pool;
 };

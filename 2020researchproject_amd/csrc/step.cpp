@@ -10,6 +10,7 @@
 // Both stop on the reference's criterion ||b - A x||_2 <= tol ||b||_2 of the
 // full padded system (pressure rows diagonal-normalised), from x0 = 0.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -630,6 +631,8 @@ void iterate_impl(IBMSolver *s, int steps, bool verbose) {
         launch_conv_rhs(s->stream, s->geo, sp, s->flag.p, u3, U3, s->p.p, Np, N, b, tp, s->partial.p);
         for (int l = 0; l < 3; l++) std::swap(s->N[l], s->Np[l]);
     }
+    s->step_ms.clear();
+    auto tprev = std::chrono::steady_clock::now();
     for (int i = 0; i < steps;) {
         do_step(s);
         i++;
@@ -637,8 +640,11 @@ void iterate_impl(IBMSolver *s, int steps, bool verbose) {
         s->time = t0 + s->dt * i;
         update_outer(s);
         update_ghost(s);
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        const auto tnow = std::chrono::steady_clock::now();
+        s->step_ms.push_back(std::chrono::duration<double, std::milli>(tnow - tprev).count());
+        tprev = tnow;
         if (verbose) {
-            HIP_CHECK(hipStreamSynchronize(s->stream));
             if (s->iter % 10 == 1 || i == 1) {
                 printf("\n  iter       u1 res       u2 res       u3 res        p res   its(u1 u2 u3 p)\n");
                 printf("-------------------------------------------------------------------------------\n");
@@ -692,5 +698,11 @@ int IBMSolver_get_kernel_stat(IBMSolver *s, int idx, char *name, int namelen, do
 }
 
 void IBMSolver_calc_tau_r(IBMSolver *s) { calc_tau(s); }
+
+int IBMSolver_get_step_times(IBMSolver *s, double *out, int max) {
+    const int n = (int)s->step_ms.size();
+    for (int i = 0; out && i < n && i < max; i++) out[i] = s->step_ms[i];
+    return n;
+}
 
 }  // extern "C"

@@ -80,6 +80,7 @@ _SIGS = {
 _OPTIONAL = {
     "IBMSolver_calc_body_force": (None, [_vp, _dp]),
     "IBMSolver_enable_timing": (None, [_vp, C.c_int]),
+    "IBMSolver_get_step_times": (C.c_int, [_vp, _dp, C.c_int]),
     "IBMSolver_reset_timing": (None, [_vp]),
     "IBMSolver_get_kernel_stat": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _dp,
                                             C.POINTER(C.c_long), _dp]),
@@ -259,6 +260,13 @@ class Solver:
 
     def reset_timing(self):
         self.lib.IBMSolver_reset_timing(self.h)
+
+    def step_times(self) -> np.ndarray:
+        n = self.lib.IBMSolver_get_step_times(self.h, None, 0)
+        out = np.zeros(n)
+        if n:
+            self.lib.IBMSolver_get_step_times(self.h, _dptr(out), n)
+        return out
 
     def kernel_stats(self) -> list[dict]:
         n = self.lib.IBMSolver_get_kernel_stat(self.h, -1, None, 0, None, None, None)

@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Benchmark of the fractional-step IBM time step on one MI355X.
+
+Workload: flow past a sphere (phi = |x| - 0.5), domain [-5,15]x[-5,5]^2,
+256x128x128 cells cosine-stretched toward the body, Re = 300, dt = 0.002,
+velocity inlet / pressure outlet / free-slip walls, solves to relative
+residual 1e-10. A step is one IBMSolver_iterate time step (AB2 convection,
+three Crank-Nicolson Helmholtz solves, pressure Poisson solve, projection,
+boundary and immersed-boundary ghost updates).
+
+The headline value is the FP32-multigrid + FP64-refinement path in
+Mcell-steps/s; the plain FP64 path runs the same steps for the speed-up,
+max|div U| and velocity agreement. Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+PKG = importlib.import_module("2020researchproject_amd")
+I, cases = PKG.ibm3d, PKG.cases
+
+METRIC = "Mcell-steps/s & p99 latency: FP32-MG+FP64 refine vs FP64, same div tol"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(prec: int, nx: int, ny: int, nz: int, steps: int, warmup: int, tol: float, timing: bool):
+    s = I.Solver()
+    t0 = time.perf_counter()
+    cases.setup_sphere(s, nx, ny, nz, tol=tol, precision=prec)
+    log(f"[bench] setup prec={prec} {nx}x{ny}x{nz}: {time.perf_counter() - t0:.2f} s, "
+        f"ghost cells {len(s.ghost_cells())}")
+    if warmup:
+        s.iterate(warmup)
+        log(f"[bench] warmup {warmup} steps done (its {s.last_iters().tolist()})")
+    if timing:
+        s.enable_timing(True)
+        s.reset_timing()
+    t0 = time.perf_counter()
+    s.iterate(steps)
+    wall = time.perf_counter() - t0
+    step_ms = s.step_times()
+    stats = s.kernel_stats() if timing else []
+    if timing:
+        s.enable_timing(False)
+    res = {
+        "wall": wall, "step_ms": step_ms, "stats": stats, "max_div": s.max_div(),
+        "iters": s.last_iters().tolist(), "norms": s.last_norms().tolist(),
+        "u": [s.get(n)[2:-2, 2:-2, 2:-2] for n in ("U1", "U2", "U3")],
+        "p": s.get("P")[2:-2, 2:-2, 2:-2],
+        "force": s.body_force().tolist(),
+    }
+    log(f"[bench] prec={prec} timed {steps} steps: {wall:.3f} s, max|div U| {res['max_div']:.3e}, "
+        f"its {res['iters']}")
+    s.close()
+    return res
+
+
+def rel_l2(a, b):
+    m = np.isfinite(a) & np.isfinite(b)
+    return float(np.linalg.norm(a[m] - b[m]) / max(np.linalg.norm(b[m]), 1e-300))
+
+
+def cpu_baseline(tol: float):
+    """Oracle (FP64 CPU restatement, OpenMP, ILU(0)-BiCGSTAB pressure) on a
+    bounded sample of the same workload: one step of the sphere case on the
+    128x64x64 grid (same domain, stretching ratio and physics)."""
+    lib = os.path.join(ROOT, "oracle", "libibm3d_oracle.so")
+    if not os.path.exists(lib):
+        return None
+    nx, ny, nz = 128, 64, 64
+    o = I.Solver(lib)
+    cases.setup_sphere(o, nx, ny, nz, tol=tol)
+    t0 = time.perf_counter()
+    o.iterate(1)
+    dt = time.perf_counter() - t0
+    o.close()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": nx * ny * nz / dt / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+            "sample": f"1 step of the sphere case on {nx}x{ny}x{nz} (1/8 of the cells), "
+                      f"FP64 oracle, {dt:.2f} s"}
+
+
+def roofline(stats, cells: float):
+    if not stats:
+        return None
+    dom = max(stats, key=lambda t: t["ms"])
+    avg_ms = dom["ms"] / max(dom["count"], 1)
+    achieved = dom["bytes"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return dom, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                 "kernel": dom["name"], "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes"]}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=str, default="256x128x128")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stats", action="store_true", help="print per-kernel timing to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU (one GPU holds the whole domain); --gpus {args.gpus} "
+              "is not supported", file=sys.stderr)
+        return 2
+    nx, ny, nz = (int(v) for v in args.grid.split("x"))
+    cells = float(nx * ny * nz)
+
+    mixed = run_gpu(I.PREC_MIXED, nx, ny, nz, args.steps, args.warmup, args.tol, timing=True)
+    value = cells * args.steps / mixed["wall"] / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(mixed["wall"] / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"sphere {nx}x{ny}x{nz} cosine-stretched, Re 300, dt 0.002, tol {args.tol:g}",
+                   "cells": int(cells), "precision": "FP32 multigrid/Krylov inside FP64 refinement"},
+        "p50_ms": round(float(np.percentile(mixed["step_ms"], 50)), 3),
+        "p99_ms": round(float(np.percentile(mixed["step_ms"], 99)), 3),
+        "max_div": mixed["max_div"],
+        "iters_last_step": mixed["iters"],
+    }
+    dom_rf = roofline(mixed["stats"], cells)
+    if dom_rf:
+        out["roofline"] = dom_rf[1]
+    if args.stats:
+        for t in sorted(mixed["stats"], key=lambda t: -t["ms"]):
+            avg = t["ms"] / max(t["count"], 1)
+            gbs = t["bytes"] / (avg * 1e-3) / 1e9 if avg > 0 and t["bytes"] else 0
+            log(f"[stats] {t['name']:18s} total {t['ms']:9.2f} ms  n {t['count']:6d}  avg {avg * 1e3:9.1f} us  "
+                f"{gbs:8.1f} GB/s")
+    if not args.no_fp64:
+        f64 = run_gpu(I.PREC_FP64, nx, ny, nz, args.steps, args.warmup, args.tol, timing=False)
+        v64 = cells * args.steps / f64["wall"] / 1e6
+        out["fp64_value"] = round(v64, 2)
+        out["speedup_vs_fp64"] = round(value / v64, 3)
+        out["fp64_max_div"] = f64["max_div"]
+        out["fp64_p99_ms"] = round(float(np.percentile(f64["step_ms"], 99)), 3)
+        out["rel_l2_u_mixed_vs_fp64"] = max(rel_l2(a, b) for a, b in zip(mixed["u"], f64["u"]))
+    if not args.no_cpu:
+        cb = cpu_baseline(args.tol)
+        if cb:
+            out["cpu_baseline"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cb.items()}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -72,6 +72,11 @@ double IBMSolver_max_div(IBMSolver *);
    (Fx, Fy, Fz); drag and lift coefficients follow by scaling. */
 void IBMSolver_calc_body_force(IBMSolver *, double out[3]);
 
+/* Wall-clock milliseconds of each time step of the last IBMSolver_iterate
+   call (host clock, the step's device work complete at its end): writes up
+   to max values, returns the number of steps. */
+int IBMSolver_get_step_times(IBMSolver *, double *out, int max);
+
 /* Per-kernel timing with HIP events on the solver's stream. */
 void IBMSolver_enable_timing(IBMSolver *, int on);
 void IBMSolver_reset_timing(IBMSolver *);

@@ -1,0 +1,260 @@
+"""Parity of the HIP path with the FP64 CPU oracle (needs an MI355X).
+
+Both libraries run the same IBMSolver_* call sequence on the same inputs;
+fields are compared in the reference's padded layout. Bars:
+  FP64 mode         rel-L2 <= 1e-8 for u, p, U at matched tolerance (1e-12)
+  mixed mode        rel-L2 <= 1e-6 (stated tolerance of the FP32-MG inside
+                    FP64-refinement path at solver tolerance 1e-10)
+  integer sets      flags and ghost-cell index sets bit-exact
+  NaN placement     identical (solid cells)
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+INNER = (slice(2, -2),) * 3
+FACE_SL = {"FACE_U1": (slice(0, None), slice(2, -2), slice(2, -2)),
+           "FACE_U2": (slice(2, -2), slice(0, None), slice(2, -2)),
+           "FACE_U3": (slice(2, -2), slice(2, -2), slice(0, None))}
+
+
+def rel_l2(a, b):
+    assert np.array_equal(np.isnan(a), np.isnan(b)), "NaN placement differs"
+    m = ~np.isnan(b)
+    den = np.linalg.norm(b[m])
+    return float(np.linalg.norm(a[m] - b[m]) / den) if den > 0 else float(np.abs(a[m]).max())
+
+
+def compare(g, o, tol, names=("U1", "U2", "U3", "P", "FACE_U1", "FACE_U2", "FACE_U3")):
+    worst = {}
+    for n in names:
+        sl = FACE_SL.get(n, INNER)
+        worst[n] = rel_l2(g.get(n)[sl], o.get(n)[sl])
+    print("rel-L2 vs oracle:", {k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
+    bad = {k: v for k, v in worst.items() if not v <= tol}
+    assert not bad, bad
+
+
+def run_pair(new_gpu, new_oracle, setup, steps, **kw):
+    g, o = new_gpu(), new_oracle()
+    for s in (g, o):
+        setup(s, **kw)
+    for s in (g, o):
+        s.iterate(steps)
+    return g, o
+
+
+def test_cavity16_fp64_parity(new_gpu, new_oracle, cases):
+    g, o = run_pair(new_gpu, new_oracle, cases.setup_cavity, 5, n=16, tol=1e-12)
+    compare(g, o, 1e-8)
+    assert g.max_div() < 1e-8
+    assert g.iter == o.iter == 5 and g.time == pytest.approx(o.time, rel=1e-15)
+
+
+def test_cavity32_fp64_parity_config1(new_gpu, new_oracle, cases):
+    # the 32^3 cavity of the CPU-runnable configuration (Re 100, dt 0.005), 20 steps
+    g, o = run_pair(new_gpu, new_oracle, cases.setup_cavity, 20, n=32, tol=1e-12)
+    compare(g, o, 1e-8)
+
+
+def test_sphere_fp64_parity_and_ghost_sets(new_gpu, new_oracle, cases):
+    g, o = run_pair(new_gpu, new_oracle, cases.setup_sphere, 4, nx=48, ny=24, nz=24, tol=1e-12)
+    assert np.array_equal(g.flags(), o.flags())
+    assert np.array_equal(g.ghost_cells(), o.ghost_cells())
+    assert len(g.ghost_cells()) > 50
+    compare(g, o, 1e-8)
+    assert g.max_div() < 1e-8
+
+
+def read_poly(lib, path):
+    import ctypes as C
+    import ctypes.util
+    libc = C.CDLL(ctypes.util.find_library("c"))
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    lib.Polyhedron_read_stl.argtypes = [C.c_void_p, C.c_void_p]
+    poly = lib.Polyhedron_new()
+    fp = libc.fopen(path.encode(), b"rb")
+    lib.Polyhedron_read_stl(poly, fp)
+    libc.fclose(fp)
+    return poly
+
+
+def test_stl_sphere_flags_bit_exact(new_gpu, new_oracle, I, cases):
+    from conftest import GOLDEN
+    stl = os.path.join(GOLDEN, "sphere.stl")
+    for faces in (cases.sphere_faces(48, 24, 24, 0.12), [cases.uniform_faces(-1.5, 1.5, 20)] * 3):
+        res = []
+        for make in (new_gpu, new_oracle):
+            s = make()
+            poly = read_poly(s.lib, stl)
+            s.set_grid(*faces)
+            s.set_params(300, 0.002)
+            s.set_bc(I.DIR_WEST, I.BC_VELOCITY_COMPONENT, I.VAL_CONST, 1.0, 0.0, 0.0)
+            s.set_bc(I.DIR_EAST, I.BC_PRESSURE, I.VAL_CONST, 0.0)
+            s.set_bc(I.DIR_NORTH | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP, I.BC_FREE_SLIP_WALL)
+            s.set_obstacle(poly)
+            s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, 1e-12)
+            s.assemble()
+            s.lib.Polyhedron_destroy(poly)
+            s.init_flow_const(1, 0, 0, 0)
+            res.append(s)
+        g, o = res
+        assert np.array_equal(g.flags(), o.flags())
+        assert np.array_equal(g.ghost_cells(), o.ghost_cells())
+        np.testing.assert_allclose(g.get("LVSET"), o.get("LVSET"), rtol=0, atol=1e-12)
+        g.iterate(2)
+        o.iterate(2)
+        compare(g, o, 1e-8, names=("U1", "U2", "U3", "P"))
+
+
+def test_cavity_mixed_precision(new_gpu, new_oracle, cases, I):
+    g, o = new_gpu(), new_oracle()
+    cases.setup_cavity(g, n=24, tol=1e-10, precision=I.PREC_MIXED)
+    cases.setup_cavity(o, n=24, tol=1e-12)
+    g.iterate(5)
+    o.iterate(5)
+    compare(g, o, 1e-6)
+    assert g.max_div() < 1e-8
+    assert (g.last_norms() <= 1e-10).all()
+
+
+def test_sphere_mixed_precision(new_gpu, new_oracle, cases, I):
+    g, o = new_gpu(), new_oracle()
+    cases.setup_sphere(g, 48, 24, 24, tol=1e-10, precision=I.PREC_MIXED)
+    cases.setup_sphere(o, 48, 24, 24, tol=1e-12)
+    g.iterate(4)
+    o.iterate(4)
+    compare(g, o, 1e-6)
+    assert g.max_div() < 1e-8
+
+
+def setup_channel(s, I, cases, model=None, coef=0.0, n=(16, 12, 8), tol=1e-12):
+    import math
+    xf = cases.uniform_faces(0.0, 2 * math.pi, n[0])
+    yf = np.array([-math.cos(math.pi * j / n[1]) for j in range(n[1] + 1)])  # cos-clustered walls
+    zf = cases.uniform_faces(0.0, math.pi, n[2])
+    s.set_grid(xf, yf, zf)
+    s.set_params(180.0, 0.002)
+    s.set_bc(I.DIR_WEST | I.DIR_EAST | I.DIR_DOWN | I.DIR_UP, I.BC_ALL_PERIODIC)
+    s.set_bc(I.DIR_NORTH | I.DIR_SOUTH, I.BC_STATIONARY_WALL)
+    s.set_ext_force(1.0, 0.0, 0.0)
+    if model is not None:
+        s.set_turb_model(model, coef)
+    s.set_obstacle(None)
+    s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, tol)
+    s.assemble()
+    xc = cases.cell_centres(xf, periodic=True)
+    yc = cases.cell_centres(yf)
+    zc = cases.cell_centres(zf, periodic=True)
+    X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+    s.init_flow_const(0, 0, 0, 0)
+    u = 15.0 * (1 - Y**2) + 1.5 * np.sin(2 * X) * np.cos(3 * Z) * (1 - Y**2)
+    v = 0.8 * np.cos(X) * np.sin(Z) * (1 - Y**2) ** 2
+    s.set("U1", u)
+    s.set("U2", v)
+    return s
+
+
+def test_channel_periodic_parity(new_gpu, new_oracle, I, cases):
+    g, o = new_gpu(), new_oracle()
+    for s in (g, o):
+        setup_channel(s, I, cases)
+        s.iterate(4)
+    compare(g, o, 1e-8)
+
+
+@pytest.mark.parametrize("model,coef", [(1, 0.1), (3, 0.07)])
+def test_channel_les_parity(new_gpu, new_oracle, I, cases, model, coef):
+    g, o = new_gpu(), new_oracle()
+    for s in (g, o):
+        setup_channel(s, I, cases, model=model, coef=coef)
+        s.iterate(3)
+    compare(g, o, 1e-8)
+    for n in ("TAU11", "TAU12", "TAU23"):
+        assert rel_l2(g.get(n)[INNER], o.get(n)[INNER]) < 1e-8
+
+
+def test_function_valued_boundary_data(new_gpu, new_oracle, I, cases):
+    def inlet_u(t, x, y, z):
+        return 1.0 + 0.1 * y * y + 0.05 * t
+
+    def inlet_v(t, x, y, z):
+        return 0.02 * z
+
+    def zero(t, x, y, z):
+        return 0.0
+
+    def outlet_p(t, x, y, z):
+        return 0.01 * y + 0.3 * t
+
+    res = []
+    for make in (new_gpu, new_oracle):
+        s = make()
+        xf, yf, zf = cases.sphere_faces(32, 16, 16, 0.12)
+        s.set_grid(xf, yf, zf)
+        s.set_params(200, 0.004)
+        s.set_bc(I.DIR_WEST, I.BC_VELOCITY_COMPONENT, I.VAL_FUNC, inlet_u, inlet_v, zero)
+        s.set_bc(I.DIR_EAST, I.BC_PRESSURE, I.VAL_FUNC, outlet_p)
+        s.set_bc(I.DIR_NORTH | I.DIR_SOUTH, I.BC_FREE_SLIP_WALL)
+        s.set_bc(I.DIR_DOWN | I.DIR_UP, I.BC_STATIONARY_WALL)
+        s.set_obstacle_sphere(0.0, 0.0, 0.0, 0.5)
+        s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, 1e-12)
+        s.assemble()
+        s.init_flow_const(1, 0, 0, 0)
+        s.iterate(3)
+        res.append(s)
+    compare(*res, 1e-8)
+
+
+def test_repeated_iterate_resets_ab2_like_reference(new_gpu, new_oracle, cases):
+    g, o = new_gpu(), new_oracle()
+    for s in (g, o):
+        cases.setup_cavity(s, n=12, tol=1e-12)
+        s.iterate(2)
+        s.iterate(3)
+    compare(g, o, 1e-8)
+    assert g.iter == 5
+
+
+def test_two_instances_are_independent(new_gpu, cases):
+    a, b, c = new_gpu(), new_gpu(), new_gpu()
+    cases.setup_cavity(a, n=12, tol=1e-12)
+    cases.setup_sphere(b, 32, 16, 16, tol=1e-12)
+    cases.setup_cavity(c, n=12, tol=1e-12)
+    a.iterate(2)
+    b.iterate(2)
+    c.iterate(2)
+    for n in ("U1", "P"):
+        assert np.array_equal(a.get(n), c.get(n))
+
+
+def test_export_and_restart_roundtrip(new_gpu, cases, tmp_path):
+    from scipy.io import netcdf_file
+    g = new_gpu()
+    cases.setup_sphere(g, 32, 16, 16, tol=1e-12)
+    g.iterate(2)
+    base = str(tmp_path / "res")
+    g.lib.IBMSolver_export_result(g.h, base.encode())
+    with netcdf_file(base + ".nc", "r", mmap=False) as nc:
+        assert nc.dimensions["x"] == 36 and nc.dimensions["z"] == 20 and nc.dimensions["time"] == 1
+        u = nc.variables["u"][:]
+        assert u.shape == (1, 20, 20, 36) and u.dtype == np.dtype(">f4")
+        assert int(nc.variables["iter"].getValue()) == 2
+        ref = g.get("U1")
+        np.testing.assert_array_equal(u[0].transpose(2, 1, 0), ref.astype(np.float32))
+        assert "vorticity" in nc.variables and "Q" in nc.variables
+    h = new_gpu()
+    cases.setup_sphere(h, 32, 16, 16, tol=1e-12)
+    h.lib.IBMSolver_init_flow_file(h.h, base.encode())
+    assert h.iter == 2 and h.time == pytest.approx(g.time)
+    np.testing.assert_array_equal(h.get("U1"), g.get("U1").astype(np.float32).astype(np.float64))
+    g.lib.IBMSolver_export_lvset_flag(g.h, (base + "_lf").encode())
+    with netcdf_file(base + "_lf.nc", "r", mmap=False) as nc:
+        assert np.array_equal(nc.variables["flag"][:].transpose(2, 1, 0), g.flags())

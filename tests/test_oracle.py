@@ -1,0 +1,400 @@
+"""Pins of the FP64 CPU oracle (CPU only).
+
+The reference ships no tests or golden vectors for this path, so every oracle
+function is pinned here by what the mathematics fixes: closed forms on small
+grids, exactness on linear fields, operator identities, brute force on tiny
+inputs, and analytic solutions. Each check is chosen so that a plausible
+restatement mistake (dropped term, wrong index, transposed weight, wrong
+spacing) fails it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def bind_oracle_hooks(lib):
+    lib.oracle_apply_matrix.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    lib.oracle_matrix_row.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _dp]
+    lib.oracle_matrix_row.restype = C.c_int
+    lib.oracle_get_kcoef.argtypes = [C.c_void_p, _dp]
+    lib.oracle_get_coords.argtypes = [C.c_void_p] + [_dp] * 6
+    for n in ("oracle_update_outer", "oracle_update_ghost", "oracle_calc_N"):
+        getattr(lib, n).argtypes = [C.c_void_p]
+    return lib
+
+
+def coords(s):
+    nx, ny, nz = s.Nx, s.Ny, s.Nz
+    arr = [np.zeros(n + 4) for n in (nx, ny, nz, nx, ny, nz)]
+    s.lib.oracle_get_coords(s.h, *[_p(a) for a in arr])
+    return arr  # xc, yc, zc, dx, dy, dz
+
+
+def kcoef(s):
+    nx, ny, nz = s.Nx, s.Ny, s.Nz
+    out = np.zeros(2 * (nx + 4) + 2 * (ny + 4) + 2 * (nz + 4))
+    s.lib.oracle_get_kcoef(s.h, _p(out))
+    o = np.cumsum([0, nx + 4, nx + 4, ny + 4, ny + 4, nz + 4, nz + 4])
+    return [out[o[a]:o[a + 1]] for a in range(6)]
+
+
+def box(I, s, xf, yf, zf, Re=2.0, dt=4.0, bc=None, tol=1e-12):
+    s.set_grid(xf, yf, zf)
+    s.set_params(Re, dt)
+    if bc is None:
+        s.set_bc(I.DIR_EAST | I.DIR_WEST | I.DIR_NORTH | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP,
+                 I.BC_STATIONARY_WALL)
+    else:
+        bc(s)
+    s.set_obstacle(None)
+    s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, tol)
+    s.assemble()
+    return s
+
+
+# ---------------------------------------------------------------------------
+# a13: grid metrics and K coefficients  <- src/ibm3d_setup.c:536-632
+# ---------------------------------------------------------------------------
+def test_kcoef_closed_form_nonuniform(I, new_oracle, oracle_lib):
+    bind_oracle_hooks(oracle_lib)
+    s = box(I, new_oracle(), [0, 1, 3, 6, 10], [0, 1, 2, 3], [0, 2, 4, 6])
+    kW, kE, kS, kN, kD, kU = kcoef(s)
+    # dt / (2 Re) = 1; dx = 1,2,3,4; xc = .5,2,4.5,8; mirrored ghosts -.5, 12
+    np.testing.assert_allclose(kW[2:6], [1 / 1, 1 / 3, 1 / 7.5, 1 / 14], rtol=1e-15)
+    np.testing.assert_allclose(kE[2:6], [1 / 1.5, 1 / 5, 1 / 10.5, 1 / 16], rtol=1e-15)
+    np.testing.assert_allclose(kS[2:5], [1, 1, 1], rtol=1e-15)
+    np.testing.assert_allclose(kU[2:5], [1 / 4, 1 / 4, 1 / 4], rtol=1e-15)
+    assert kW[0] == kW[1] == kW[6] == kW[7] == 0  # outer entries stay zero
+
+
+def test_ghost_widths_periodic(I, new_oracle, oracle_lib):
+    bind_oracle_hooks(oracle_lib)
+
+    def per(s):
+        s.set_bc(I.DIR_EAST | I.DIR_WEST, I.BC_ALL_PERIODIC)
+        s.set_bc(I.DIR_NORTH | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP, I.BC_STATIONARY_WALL)
+    s = box(I, new_oracle(), [0, 1, 3, 6, 10], [0, 1, 2, 3], [0, 1, 2, 3], bc=per)
+    xc, yc, zc, dx, dy, dz = coords(s)
+    np.testing.assert_allclose(dx, [3, 4, 1, 2, 3, 4, 1, 2])   # wrapped widths
+    np.testing.assert_allclose(xc[:2], [-5.5, -2.0])           # 0.5 - (4+1)/2, -2 - (3+4)/2
+    np.testing.assert_allclose(xc[6:], [10.5, 12.0])
+    kW, kE = kcoef(s)[:2]
+    assert kW[2] == pytest.approx(1 / 2.5, rel=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# a11: mirror-point trilinear stencil  <- src/utils.c:111-171
+# ---------------------------------------------------------------------------
+def sphere_solver(I, s, cases, n=(24, 20, 20)):
+    return cases.setup_sphere(s, *n, tol=1e-10)
+
+
+def test_ghost_interp_reproduces_mirror_point(I, new_oracle, cases):
+    s = sphere_solver(I, new_oracle(), cases)
+    xc, yc, zc, *_ = coords(bind_oracle_hooks(s.lib) and s)
+    lv = s.get("LVSET")
+    ghosts = s.ghost_cells()
+    assert len(ghosts) > 20
+    idx = np.zeros((8, 3), dtype=np.int32)
+    w = np.zeros(8)
+    for (i, j, k) in ghosts:
+        s.lib.IBMSolver_ghost_interp(s.h, int(i), int(j), int(k), idx.ctypes.data_as(_ip), _p(w))
+        L = lambda a, b, c: lv[a + 2, b + 2, c + 2]
+        n = np.array([(L(i + 1, j, k) - L(i - 1, j, k)) / (xc[i + 3] - xc[i + 1]),
+                      (L(i, j + 1, k) - L(i, j - 1, k)) / (yc[j + 3] - yc[j + 1]),
+                      (L(i, j, k + 1) - L(i, j, k - 1)) / (zc[k + 3] - zc[k + 1])])
+        m = np.array([xc[i + 2], yc[j + 2], zc[k + 2]]) - 2 * L(i, j, k) * n
+        pts = np.array([[xc[a + 2], yc[b + 2], zc[c + 2]] for a, b, c in idx])
+        assert w.sum() == pytest.approx(1.0, abs=1e-12)
+        # trilinear weights reproduce every linear function at the mirror point
+        np.testing.assert_allclose(w @ pts, m, atol=1e-12)
+        # bit order x=4, y=2, z=1 and the base cell brackets the mirror point
+        assert np.array_equal(idx[7] - idx[0], [1, 1, 1]) and np.array_equal(idx[4] - idx[0], [1, 0, 0])
+        assert pts[0][0] <= m[0] < pts[7][0] and pts[0][1] <= m[1] < pts[7][1]
+        assert (w >= -1e-12).all()
+        # mirror point lies on the fluid side, about as far out as the ghost is in
+        assert np.linalg.norm(m) > 0.5
+
+
+def test_flags_rule_and_invariants(I, new_oracle, cases):
+    s = sphere_solver(I, new_oracle(), cases)
+    lv, fl = s.get("LVSET"), s.flags()
+    assert ((lv > 0) == (fl == I.FLAG_FLUID)).all()
+    fluid = fl == I.FLAG_FLUID
+    nb_fluid = np.zeros_like(fluid)
+    for ax in range(3):
+        for sh in (1, -1):
+            r = np.roll(fluid, sh, axis=ax)
+            sl = [slice(None)] * 3
+            sl[ax] = 0 if sh == 1 else -1
+            r[tuple(sl)] = False
+            nb_fluid |= r
+    assert (fl[(~fluid) & nb_fluid] == I.FLAG_GHOST).all()
+    assert (fl[(~fluid) & ~nb_fluid] == I.FLAG_SOLID).all()
+    # analytic level set: phi = |x| - 0.5 at the padded centres
+    xc, yc, zc, *_ = coords(bind_oracle_hooks(s.lib) and s)
+    X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+    np.testing.assert_allclose(lv, np.sqrt(X**2 + Y**2 + Z**2) - 0.5, rtol=0, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# a5 / a8: assembled operators  <- src/ibm3d_setup.c:1566-2908
+# ---------------------------------------------------------------------------
+def test_operator_row_identities(I, new_oracle, cases):
+    s = sphere_solver(I, new_oracle(), cases)
+    bind_oracle_hooks(s.lib)
+    fl = s.flags().ravel()
+    P = fl.size
+    ones = np.ones(P)
+    y = np.zeros(P)
+    inner = np.zeros(s.shape, dtype=bool)
+    inner[2:-2, 2:-2, 2:-2] = True
+    inner = inner.ravel()
+    for t in (1, 2, 3):
+        s.lib.oracle_apply_matrix(s.h, t, _p(ones), _p(y))
+        np.testing.assert_allclose(y[inner & (fl == I.FLAG_FLUID)], 1.0, atol=1e-14)   # (1+sumK) - sumK
+        np.testing.assert_allclose(y[inner & (fl == I.FLAG_GHOST)], 2.0, atol=1e-12)   # (1+ws) + (1-ws)
+        np.testing.assert_allclose(y[inner & (fl == I.FLAG_SOLID)], 1.0, atol=0)
+    s.lib.oracle_apply_matrix(s.h, 4, _p(ones), _p(y))
+    np.testing.assert_allclose(y[inner & (fl != I.FLAG_SOLID)], 0.0, atol=1e-13)        # Neumann / mirror rows
+
+
+def test_momentum_rows_symmetric_after_volume_scaling(I, new_oracle):
+    s = new_oracle()
+    bind_oracle_hooks(s.lib)
+    box(I, s, [0, .3, .5, 1.1, 1.4, 2.0], [0, .2, .7, 1.0], [0, .5, .6, 1.3])
+    xc, yc, zc, dx, dy, dz = coords(s)
+    Ny4, Nz4 = s.Ny + 4, s.Nz + 4
+    row = lambda i, j, k: ((i + 2) * Ny4 + j + 2) * Nz4 + k + 2
+    cols = np.zeros(9, dtype=np.int32)
+    vals = np.zeros(9)
+    A = {}
+    for i in range(s.Nx):
+        for j in range(s.Ny):
+            for k in range(s.Nz):
+                n = s.lib.oracle_matrix_row(s.h, 1, row(i, j, k), cols.ctypes.data_as(_ip), _p(vals))
+                A[row(i, j, k)] = dict(zip(cols[:n].tolist(), vals[:n].tolist()))
+    vol = lambda i, j, k: dx[i + 2] * dy[j + 2] * dz[k + 2]
+    checked = 0
+    for i in range(s.Nx - 1):
+        for j in range(s.Ny):
+            for k in range(s.Nz):
+                a, b = row(i, j, k), row(i + 1, j, k)
+                assert vol(i, j, k) * A[a][b] == pytest.approx(vol(i + 1, j, k) * A[b][a], rel=1e-13)
+                checked += 1
+    assert checked > 20
+
+
+# ---------------------------------------------------------------------------
+# a2: convection  <- src/ibm3d_fracstep.c:117-170
+# ---------------------------------------------------------------------------
+def test_convection_exact_for_linear_field(I, new_oracle, cases):
+    s = new_oracle()
+    bind_oracle_hooks(s.lib)
+    f = cases.uniform_faces(0.0, 1.0, 8)
+    box(I, s, f, f, f, Re=100.0, dt=0.01)
+    s.init_flow_const(0, 0, 0, 0)
+    xc = coords(s)[0]
+    a, b = 0.3, 1.7
+    u1 = np.broadcast_to((a + b * xc)[:, None, None], s.shape).copy()
+    s.set("U1", u1)
+    s.lib.oracle_calc_N(s.h)
+    N1, N2 = s.get("N1")[2:-2, 2:-2, 2:-2], s.get("N2")[2:-2, 2:-2, 2:-2]
+    # N1 = d(u u)/dx = 2 u du/dx exactly for a linear u on a uniform grid
+    np.testing.assert_allclose(N1, 2 * b * (a + b * xc[2:-2])[:, None, None] * np.ones_like(N1), rtol=1e-12)
+    assert np.abs(N2).max() == 0.0
+    # uniform flow carries no convective flux divergence
+    s.set("U1", np.full(s.shape, 0.7))
+    s.set("U2", np.full(s.shape, -0.2))
+    s.lib.oracle_calc_N(s.h)
+    assert np.abs(s.get("N1")[2:-2, 2:-2, 2:-2]).max() < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# a10: outer ghost layers  <- src/ibm3d_fracstep.c:855-1906
+# ---------------------------------------------------------------------------
+def test_outer_layers_linear_identities(I, new_oracle, cases):
+    s = new_oracle()
+    bind_oracle_hooks(s.lib)
+    xf = np.array([0, .1, .25, .45, .7, 1.0, 1.4])
+
+    def bcs(s):
+        s.set_bc(I.DIR_WEST, I.BC_VELOCITY_COMPONENT, I.VAL_CONST, 0.25, -0.5, 1.0)
+        s.set_bc(I.DIR_EAST, I.BC_PRESSURE, I.VAL_CONST, 0.75)
+        s.set_bc(I.DIR_NORTH | I.DIR_SOUTH, I.BC_FREE_SLIP_WALL)
+        s.set_bc(I.DIR_DOWN, I.BC_PRESSURE, I.VAL_CONST, 0.0)
+        s.set_bc(I.DIR_UP, I.BC_STATIONARY_WALL)
+    box(I, s, xf, [0, .2, .5, .9], [0, .3, .5, 1.0], bc=bcs)
+    xc, yc, zc, dx, dy, dz = coords(s)
+    s.init_flow_const(0, 0, 0, 0)
+    lin = lambda c: (1.0 + 2.0 * c)
+    for name in ("U1", "U2", "U3"):
+        s.set(name, np.broadcast_to(lin(xc)[:, None, None], s.shape).copy())
+    s.set("P", np.broadcast_to(lin(zc)[None, None, :], s.shape).copy())
+    s.lib.oracle_update_outer(s.h)
+    u1, u2, p = s.get("U1"), s.get("U2"), s.get("P")
+    J, K = slice(2, -2), slice(2, -2)
+    # East pressure outlet: velocity linearly extrapolated -> exact for linear u
+    np.testing.assert_allclose(u1[-2, J, K], lin(xc[-2]), rtol=1e-13)
+    np.testing.assert_allclose(u1[-1, J, K], lin(xc[-1]), rtol=1e-13)
+    # West velocity inlet: interpolation to the face equals the BC value
+    ao, bi = dx[1] / 2, dx[2] / 2
+    np.testing.assert_allclose((bi * u2[1, J, K] + ao * u2[2, J, K]) / (ao + bi), -0.5, rtol=1e-13)
+    ao2, bi2 = dx[0] / 2 + dx[1], dx[2] + dx[3] / 2
+    np.testing.assert_allclose((bi2 * u1[0, J, K] + ao2 * u1[3, J, K]) / (ao2 + bi2), 0.25, rtol=1e-13)
+    # free slip (y): tangential components copied, normal component reflected
+    np.testing.assert_allclose(u1[2:-2, 1, K], u1[2:-2, 2, K])
+    # Down pressure BC on p uses the x spacing (src/ibm3d_fracstep.c:1792-1800)
+    a, b = dx[1] / 2, dx[2] / 2
+    np.testing.assert_allclose(p[2:-2, 2:-2, 1], (a + b) / b * 0.0 - a / b * p[2:-2, 2:-2, 2], rtol=1e-13)
+    # stationary wall (Up): Neumann p
+    np.testing.assert_allclose(p[2:-2, 2:-2, -2], p[2:-2, 2:-2, -3])
+
+
+# ---------------------------------------------------------------------------
+# a1: full step on analytic flows
+# ---------------------------------------------------------------------------
+def taylor_green_error(I, s, n, steps, nu=0.05, dt=0.01):
+    L = 2 * math.pi
+    f = np.array([L * i / n for i in range(n + 1)])
+    s.set_grid(f, f, f)
+    s.set_params(1.0 / nu, dt)
+    s.set_bc(0x3f, I.BC_ALL_PERIODIC)
+    s.set_obstacle(None)
+    s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, 1e-12)
+    s.assemble()
+    s.init_flow_const(0, 0, 0, 0)
+    xc = (f[1:] + f[:-1]) / 2
+    h = L / n
+    xp = np.concatenate([[xc[0] - 2 * h, xc[0] - h], xc, [xc[-1] + h, xc[-1] + 2 * h]])
+    X, Y, Z = np.meshgrid(xp, xp, xp, indexing="ij")
+    s.set("U1", np.sin(X) * np.cos(Y))
+    s.set("U2", -np.cos(X) * np.sin(Y))
+    s.set("P", (np.cos(2 * X) + np.cos(2 * Y)) / 4)
+    s.iterate(steps)
+    t = steps * dt
+    exact = np.sin(X) * np.cos(Y) * math.exp(-2 * nu * t)
+    inner = (slice(2, -2),) * 3
+    return np.abs(s.get("U1")[inner] - exact[inner]).max() / math.exp(-2 * nu * t)
+
+
+def test_taylor_green_decay_second_order(I, new_oracle):
+    e8 = taylor_green_error(I, new_oracle(), 8, 10)
+    e16 = taylor_green_error(I, new_oracle(), 16, 10)
+    assert e16 < 0.05
+    assert 3.0 < e8 / e16 < 5.5  # spatial order 2
+
+
+# ---------------------------------------------------------------------------
+# a14: STL reader and closest-point transform  <- src/geo3d.c:229-704
+# ---------------------------------------------------------------------------
+def read_stl_triangles(path):
+    d = open(path, "rb").read()
+    n = int(np.frombuffer(d[80:84], dtype="<u4")[0])
+    rec = np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")])
+    r = np.frombuffer(d[84:84 + n * 50], dtype=rec)
+    return r["v"].astype(np.float64), r["n"].astype(np.float64)
+
+
+def point_triangle_distance(p, tri):
+    """Exact distance from points p (M,3) to triangles tri (T,3,3): min over
+    the interior projection, the three edges and the three vertices."""
+    a, b, c = tri[:, 0], tri[:, 1], tri[:, 2]
+    n = np.cross(b - a, c - a)
+    nn = np.linalg.norm(n, axis=1)
+    nh = n / nn[:, None]
+    d = p[:, None, :] - a[None]
+    t = np.einsum("mtk,tk->mt", d, nh)
+    q = p[:, None, :] - t[..., None] * nh[None]
+    def inside(q):
+        c1 = np.einsum("mtk,tk->mt", np.cross(b - a, q - a), n)
+        c2 = np.einsum("mtk,tk->mt", np.cross(c - b, q - b), n)
+        c3 = np.einsum("mtk,tk->mt", np.cross(a - c, q - c), n)
+        return (c1 >= 0) & (c2 >= 0) & (c3 >= 0)
+    best = np.where(inside(q), np.abs(t), np.inf)
+    for u, v in ((a, b), (b, c), (c, a)):
+        e = v - u
+        s = np.clip(np.einsum("mtk,tk->mt", p[:, None, :] - u[None], e) / np.einsum("tk,tk->t", e, e), 0, 1)
+        h = u[None] + s[..., None] * e[None]
+        best = np.minimum(best, np.linalg.norm(p[:, None, :] - h, axis=2))
+    return best.min(axis=1)
+
+
+def test_stl_reader_counts(I, oracle_lib):
+    import ctypes.util
+    libc = C.CDLL(ctypes.util.find_library("c"))
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    oracle_lib.Polyhedron_read_stl.argtypes = [C.c_void_p, C.c_void_p]
+    oracle_lib.oracle_poly_counts.argtypes = [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    oracle_lib.oracle_poly_counts.restype = C.c_size_t
+    poly = oracle_lib.Polyhedron_new()
+    fp = libc.fopen(os.path.join(GOLDEN, "sphere.stl").encode(), b"rb")
+    oracle_lib.Polyhedron_read_stl(poly, fp)
+    libc.fclose(fp)
+    ne, nv = C.c_size_t(), C.c_size_t()
+    nf = oracle_lib.oracle_poly_counts(poly, C.byref(ne), C.byref(nv))
+    assert (nf, ne.value, nv.value) == (528, 792, 266)   # closed: V - E + F = 2
+    oracle_lib.Polyhedron_destroy(poly)
+
+
+def test_cpt_matches_brute_force_distance(I, new_oracle, cases, oracle_lib):
+    import ctypes.util
+    libc = C.CDLL(ctypes.util.find_library("c"))
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    oracle_lib.Polyhedron_read_stl.argtypes = [C.c_void_p, C.c_void_p]
+    poly = oracle_lib.Polyhedron_new()
+    fp = libc.fopen(os.path.join(GOLDEN, "sphere.stl").encode(), b"rb")
+    oracle_lib.Polyhedron_read_stl(poly, fp)
+    libc.fclose(fp)
+    s = new_oracle()
+    f = cases.uniform_faces(-0.9, 0.9, 12)
+    s.set_grid(f, f, f)
+    s.set_params(100, 0.01)
+    s.set_bc(0x3f, I.BC_STATIONARY_WALL)
+    s.set_obstacle(poly)
+    s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, 1e-8)
+    s.assemble()
+    oracle_lib.Polyhedron_destroy(poly)
+    lv = s.get("LVSET")
+    xc, yc, zc, *_ = coords(bind_oracle_hooks(s.lib) and s)
+    X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    tri, nrm = read_stl_triangles(os.path.join(GOLDEN, "sphere.stl"))
+    dist = point_triangle_distance(P, tri)
+    # convex closed mesh: a point is inside iff it is behind every face plane
+    side = np.einsum("mtk,tk->mt", P[:, None, :] - tri[None, :, 0], nrm)
+    inside = (side <= 0).all(axis=1)
+    sd = np.where(inside, -dist, dist)
+    np.testing.assert_allclose(lv.ravel(), sd, atol=1e-6)
+    assert ((lv.ravel() > 0) == (sd > 0)).all()
+
+
+# ---------------------------------------------------------------------------
+# regression fixture written by tests/golden/make_golden.py
+# ---------------------------------------------------------------------------
+def test_cavity_golden_fixture(I, new_oracle, cases):
+    path = os.path.join(GOLDEN, "cavity8_5steps.npz")
+    g = np.load(path)
+    s = new_oracle()
+    cases.setup_cavity(s, n=8, tol=1e-12)
+    s.iterate(5)
+    for name in ("U1", "U2", "U3", "P"):
+        np.testing.assert_allclose(s.get(name), g[name], rtol=0, atol=1e-10)
