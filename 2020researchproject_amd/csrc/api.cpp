@@ -63,6 +63,7 @@ void IBMSolver_destroy(IBMSolver *s) {
     for (auto &t : s->timers)
         for (auto &e : t.pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (auto e : s->event_pool) (void)hipEventDestroy(e);
+    for (auto &g : s->vgraphs) (void)hipGraphExecDestroy(g.exec);
     (void)hipHostFree(s->ks_h);
     (void)hipHostFree(s->sums_h);
     (void)hipStreamDestroy(s->stream);
@@ -399,8 +400,10 @@ static void build_ghosts(IBMSolver *s) {
         G->cell.upload(cell); G->src.upload(src);
         G->w64.upload(w64); G->ws64.upload(ws64);
         G->w32.upload(w32); G->ws32.upload(ws32);
-        G->G64 = {ng, G->cell.p, G->src.p, G->w64.p, G->ws64.p};
-        G->G32 = {ng, G->cell.p, G->src.p, G->w32.p, G->ws32.p};
+        G->t64.alloc(std::max(ng, 1));
+        G->t32.alloc(std::max(ng, 1));
+        G->G64 = {ng, G->cell.p, G->src.p, G->w64.p, G->ws64.p, G->t64.p};
+        G->G32 = {ng, G->cell.p, G->src.p, G->w32.p, G->ws32.p, G->t32.p};
     }
 }
 

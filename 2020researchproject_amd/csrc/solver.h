@@ -65,8 +65,8 @@ struct SysData {
 
 struct GhostData {
     DBuf<long> cell, src;
-    DBuf<double> w64, ws64;
-    DBuf<float> w32, ws32;
+    DBuf<double> w64, ws64, t64;
+    DBuf<float> w32, ws32, t32;
     GhostRows<double> G64{};
     GhostRows<float> G32{};
 };
@@ -94,6 +94,12 @@ struct KTimer {
     double total_ms = 0;
     long count = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct VGraph {
+    const void *in;
+    void *out;
+    hipGraphExec_t exec;
 };
 
 struct Krylov {
@@ -166,6 +172,9 @@ struct _ibm_solver {
     ibm::SysData sys[4];                  // u1, u2, u3, p
     long pin = -1;
     std::vector<ibm::MGLevel> mg;         // mg[0] = fine pressure level
+    ibm::DBuf<ibm::LevelDev<double>> mg_lv64;
+    ibm::DBuf<ibm::LevelDev<float>> mg_lv32;
+    std::vector<ibm::VGraph> vgraphs;     // captured V-cycles per (input, output)
     ibm::Krylov kry;
     ibm::DBuf<double> ref_r64;            // mixed mode: FP64 defect
     ibm::DBuf<float> ref_rhs32, ref_e32;  // mixed mode: FP32 defect and correction

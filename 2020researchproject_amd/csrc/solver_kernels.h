@@ -14,6 +14,7 @@ struct GhostRows {
     const long *src;    // 8 * ng (-1 = unused)
     const T *w;         // 8 * ng
     const T *wself;     // ng
+    T *tmp;             // ng: new ghost values of one relaxation, scattered after the gather
 };
 
 // Device-resident scalars of BiCGSTAB.
@@ -96,5 +97,24 @@ void launch_prolong(hipStream_t st, const Dims &df, const Dims &dc, const MGTran
 template <typename T>
 void launch_coarse_solve(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b, T *x,
                          int sweeps, int singular, const double *dxyz);
+
+// One multigrid level as the single-workgroup tail V-cycle sees it. `tr`
+// is the transfer from the next finer level to this one.
+template <typename T>
+struct LevelDev {
+    Dims d;
+    SysCoef<T> S;
+    const uint8_t *flag;
+    T *x, *b, *r;
+    const double *dx, *dy, *dz;  // padded 1-D widths (index i + 2)
+    MGTransfer tr;
+};
+
+// V-cycle over levels l0 .. nl-1 of `lv` in one workgroup (levels small
+// enough that launch latency, not bandwidth, would dominate). b of level l0
+// must be set; writes x of level l0.
+template <typename T>
+void launch_tail_vcycle(hipStream_t st, const LevelDev<T> *lv, int l0, int nl, int pre, int post, int coarse_sweeps,
+                        int singular);
 
 }  // namespace ibm

@@ -163,6 +163,10 @@ static void read_ks(IBMSolver *s) {
 // ---------------------------------------------------------------------------
 // multigrid hierarchy of the pressure system
 // ---------------------------------------------------------------------------
+// coarsening stops at this many cells; levels at or below kTailCells run in
+// one workgroup (launch latency, not bandwidth, bounds them)
+static const long kCoarsestCells = 16, kTailCells = 4096;
+
 static void level_geometry(IBMSolver *s, MGLevel &L, const std::vector<double> &xf, const std::vector<double> &yf,
                            const std::vector<double> &zf) {
     const std::vector<double> *f[3] = {&xf, &yf, &zf};
@@ -218,8 +222,8 @@ void build_mg(IBMSolver *s) {
         const MGLevel &P = s->mg.back();
         const int n[3] = {P.d.nx, P.d.ny, P.d.nz};
         int c[3];
-        for (int a = 0; a < 3; a++) c[a] = (n[a] % 2 == 0 && n[a] >= 4) ? 1 : 0;
-        if ((!c[0] && !c[1] && !c[2]) || (long)n[0] * n[1] * n[2] <= 512) break;
+        for (int a = 0; a < 3; a++) c[a] = (n[a] % 2 == 0 && n[a] >= 2) ? 1 : 0;
+        if ((!c[0] && !c[1] && !c[2]) || (long)n[0] * n[1] * n[2] <= kCoarsestCells) break;
         // coarse faces: every other face on coarsened axes
         auto coarsen = [](const std::vector<double> &f, int cf) {
             if (!cf) return f;
@@ -308,6 +312,18 @@ void build_mg(IBMSolver *s) {
     // fine-level residual scratch
     s->mg[0].r64.alloc(s->d.ncell);
     if (s->precision == IBM3D_PREC_MIXED) s->mg[0].r32.alloc(s->d.ncell);
+    // level descriptors of the single-workgroup tail V-cycle (levels >= 1)
+    const int nl = (int)s->mg.size();
+    std::vector<LevelDev<double>> l64(nl);
+    std::vector<LevelDev<float>> l32(nl);
+    for (int l = 1; l < nl; l++) {
+        MGLevel &L = s->mg[l];
+        const double *dx = L.dxyz.p, *dy = dx + L.d.nx + 4, *dz = dy + L.d.ny + 4;
+        l64[l] = {L.d, L.sys.S64, L.flag.p, L.x64.p, L.b64.p, L.r64.p, dx, dy, dz, L.tr};
+        l32[l] = {L.d, L.sys.S32, L.flag.p, L.x32.p, L.b32.p, L.r32.p, dx, dy, dz, L.tr};
+    }
+    s->mg_lv64.upload(l64);
+    s->mg_lv32.upload(l32);
 }
 
 template <typename T> static DBuf<T> &lev_r(MGLevel &L);
@@ -329,17 +345,27 @@ template <typename T> static DBuf<T> *kvec(Krylov &K);
 template <> DBuf<double> *kvec<double>(Krylov &K) { return K.v64; }
 template <> DBuf<float> *kvec<float>(Krylov &K) { return K.v32; }
 
-static const int kPre = 2, kPost = 2, kCoarse = 100;
+static const int kPre = 2, kPost = 2, kCoarse = 40;
 
 // x = V-cycle approximation of A^-1 b on level l (x0 = 0)
+template <typename T> static const LevelDev<T> *lvdev(IBMSolver *s);
+template <> const LevelDev<double> *lvdev<double>(IBMSolver *s) { return s->mg_lv64.p; }
+template <> const LevelDev<float> *lvdev<float>(IBMSolver *s) { return s->mg_lv32.p; }
+
 template <typename T>
 static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
     MGLevel &L = s->mg[l];
     const bool fine = l == 0;
     const SysCoef<T> &S = fine ? sysc<T>(s->sys[3]) : sysc<T>(L.sys);
-    const GhostRows<T> G0 = fine ? grows<T>(s->gp) : GhostRows<T>{0, nullptr, nullptr, nullptr, nullptr};
+    const GhostRows<T> G0 = fine ? grows<T>(s->gp) : GhostRows<T>{0, nullptr, nullptr, nullptr, nullptr, nullptr};
     const uint8_t *flag = fine ? s->flag.p : L.flag.p;
     const double cell_b = (double)L.d.nx * L.d.ny * L.d.nz * sizeof(T);
+    if (l >= 1 && (long)L.d.nx * L.d.ny * L.d.nz <= kTailCells) {
+        // b of level l is in place (written by the restriction)
+        TIMED("mg_tail", 0, launch_tail_vcycle<T>(s->stream, lvdev<T>(s), l, (int)s->mg.size(), kPre, kPost, kCoarse,
+                                                  s->p_singular ? 1 : 0));
+        return;
+    }
     if (l + 1 == (int)s->mg.size()) {
         TIMED("mg_coarse", 0, launch_coarse_solve<T>(s->stream, L.d, S, flag, b, x, kCoarse, s->p_singular ? 1 : 0,
                                                      L.dxyz.p));
@@ -362,6 +388,32 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
           launch_prolong<T>(s->stream, L.d, Cl.d, Cl.tr, flag, Cl.flag.p, lev_x<T>(Cl).p, x));
     TIMED(fine ? "mg_smooth_fine" : "mg_smooth", 2 * kPost * 3 * cell_b / 2,
           launch_rbgs<T>(s->stream, L.d, S, G0, flag, b, x, kPost));
+}
+
+// V-cycle as a preconditioner. Without per-kernel timing the cycle for a
+// given (input, output) pair is captured once into a HIP graph and replayed:
+// the coarse levels are launch-latency bound, and a graph replay dispatches
+// its kernels back to back without host launch overhead.
+template <typename T>
+static void mg_precond(IBMSolver *s, const T *in, T *out) {
+    if (s->timing) {
+        vcycle<T>(s, 0, in, out);
+        return;
+    }
+    for (auto &g : s->vgraphs)
+        if (g.in == (const void *)in && g.out == (void *)out) {
+            HIP_CHECK(hipGraphLaunch(g.exec, s->stream));
+            return;
+        }
+    hipGraph_t graph;
+    HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    vcycle<T>(s, 0, in, out);
+    HIP_CHECK(hipStreamEndCapture(s->stream, &graph));
+    hipGraphExec_t exec;
+    HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(graph));
+    s->vgraphs.push_back({(const void *)in, (void *)out, exec});
+    HIP_CHECK(hipGraphLaunch(exec, s->stream));
 }
 
 // ---------------------------------------------------------------------------
@@ -393,7 +445,7 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         if (prec == 1) {
             TIMED("jacobi", 2 * cb, launch_jacobi<T>(s->stream, d, S, G, flag, in, out));
         } else if (prec == 2) {
-            vcycle<T>(s, 0, in, out);
+            mg_precond<T>(s, in, out);
         } else {
             launch_copy<T>(s->stream, d, in, out);
         }
