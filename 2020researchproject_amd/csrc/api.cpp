@@ -183,8 +183,19 @@ void IBMSolver_set_autosave(IBMSolver *s, const char *filename, int period) {
 
 void IBMSolver_set_precision(IBMSolver *s, int mode) {
     if (s->assembled) fatal("IBMSolver_set_precision must precede IBMSolver_assemble");
-    if (mode != IBM3D_PREC_FP64 && mode != IBM3D_PREC_MIXED) fatal("unknown precision mode");
+    if (mode != IBM3D_PREC_FP64 && mode != IBM3D_PREC_MIXED && mode != IBM3D_PREC_MIXED_KRYLOV)
+        fatal("unknown precision mode");
     s->precision = mode;
+}
+
+void IBMSolver_set_mg_params(IBMSolver *s, double gamma, int pre, int post, int smoother) {
+    if (s->assembled) fatal("IBMSolver_set_mg_params must precede IBMSolver_assemble");
+    if (pre < 0 || post < 0 || pre + post == 0 || pre > 16 || post > 16) fatal("invalid multigrid sweep counts");
+    if (smoother != 0 && smoother != 1) fatal("multigrid smoother must be 0 or 1");
+    s->mg_gamma = gamma;
+    s->mg_pre = pre;
+    s->mg_post = post;
+    s->mg_smoother = smoother;
 }
 
 // Mirror point and trilinear weights  <- src/utils.c:111-171 (base indices
@@ -460,6 +471,7 @@ void build_system(const IBMSolver *s, const std::vector<double> *D[3], const std
         }
         const std::vector<double> *arr[4] = {&cm, &cp, &dm, &dp};
         for (int c = 0; c < 4; c++) {
+            out.h[a][c] = *arr[c];
             out.c64[a][c].upload(*arr[c]);
             std::vector<float> f(arr[c]->begin(), arr[c]->end());
             out.c32[a][c].upload(f);
@@ -502,8 +514,9 @@ static void alloc_fields(IBMSolver *s) {
     if (s->turb_type != TURBMODEL_NONE)
         for (int a = 0; a < 6; a++) s->tau[a].alloc(n);
     for (int v = 0; v < 8; v++) s->kry.v64[v].alloc(n);
-    if (s->precision == IBM3D_PREC_MIXED) {
-        for (int v = 0; v < 8; v++) s->kry.v32[v].alloc(n);
+    if (s->precision != IBM3D_PREC_FP64) {
+        if (s->precision == IBM3D_PREC_MIXED)
+            for (int v = 0; v < 8; v++) s->kry.v32[v].alloc(n);
         s->ref_r64.alloc(n);
         s->ref_rhs32.alloc(n);
         s->ref_e32.alloc(n);

@@ -84,6 +84,8 @@ struct SysCoef {
     long pin;       // linear index of a pinned cell (identity row) or -1
     int is_p;       // pressure-type ghost rows ((1 - wself) x - sum w x = 0)
     int skip;       // drop faces to F_SOLID interior neighbours (coarse levels)
+    const uint8_t *fmask;  // skip: per cell, bit 2a / 2a+1 set when the minus / plus
+                           // neighbour along axis a is an interior F_SOLID cell
 };
 
 }  // namespace ibm

@@ -57,6 +57,7 @@ struct HostBC {
 struct SysData {
     DBuf<double> c64[3][4];   // cm, cp, dm, dp per axis
     DBuf<float> c32[3][4];
+    std::vector<double> h[3][4];  // host copies of c64
     SysCoef<double> S64{};
     SysCoef<float> S32{};
     double gamma[3][2] = {};  // RHS fold factor per face
@@ -74,18 +75,24 @@ struct GhostData {
 // One multigrid level of the pressure system.
 struct MGLevel {
     Dims d{};
-    int cx = 0, cy = 0, cz = 0;       // axes coarsened relative to the finer level
     std::vector<double> dx, dy, dz, xc, yc, zc;  // padded (+2)
     std::vector<uint8_t> hflag;
     DBuf<uint8_t> flag;
     DBuf<double> dxyz;                 // dx | dy | dz padded
     SysData sys;
-    DBuf<double> x64, b64, r64;
-    DBuf<float> x32, b32, r32;
+    DBuf<double> x64, b64, r64, xa64;  // xa: second buffer of the out-of-place smoother
+    DBuf<float> x32, b32, r32, xa32;
     // transfer to this level from the finer one (indices over the finer grid)
-    DBuf<int> pi, pj, pk;
+    DBuf<int> ci, cj, ck, si, sj, sk, pi, pj, pk;
     DBuf<double> wi, wj, wk;
+    DBuf<uint8_t> fmask;               // faces to inactive neighbours (SysCoef::fmask)
     MGTransfer tr{};
+    // exact solve of the coarsest level: dense inverse (row-major, rows in
+    // x-fastest cell order), padded index and volume of each row's cell
+    int dn = 0, dpin = -1;
+    DBuf<double> dinv64, dvol;
+    DBuf<float> dinv32;
+    DBuf<long> drow;
 };
 
 struct KTimer {
@@ -136,6 +143,9 @@ struct _ibm_solver {
     int precision = IBM3D_PREC_FP64;
     bool assembled = false;
     bool p_singular = true;
+    double mg_gamma = 3.0;       // coarsening threshold (multiples of the level's smallest width)
+    int mg_pre = 2, mg_post = 2; // smoothing sweeps per level
+    int mg_smoother = 1;         // 0 fused red-black sweeps, 1 one launch per colour
 
     // ---- host mirrors ----
     std::vector<double> lvset_h;     // reference layout (k fastest)
@@ -182,7 +192,8 @@ struct _ibm_solver {
     // diagnostics
     double final_norm[4] = {0, 0, 0, 0};
     int final_iter[4] = {0, 0, 0, 0};
-    bool timing = false;
+    int timing = 0;             // 0 off, 1 every kernel (eager), 2 kernels + whole V-cycle graphs
+    bool graphs = true;         // replay multigrid cycles as HIP graphs
     std::vector<double> step_ms;          // wall time of each step of the last iterate call
     std::vector<ibm::KTimer> timers;
     std::vector<hipEvent_t> event_pool;

@@ -48,6 +48,19 @@ template <typename T>
 void launch_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
                    const T *r, T *z);
 
+// one colour of red-black Gauss-Seidel on the fluid rows (cells with
+// (i + j + k) % 2 == color)
+template <typename T>
+void launch_rbgs_color(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b, T *x,
+                       int color, int zero_in);
+// one full red-black sweep out of place: xb = sweep(xa) (zero_in: xa = 0);
+// cells outside the fluid/solid rows are copied
+template <typename T>
+void launch_rbgs_fused(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
+                       const T *xa, T *xb, int zero_in);
+// Jacobi relaxation of the ghost rows (gather into G.tmp, then scatter)
+template <typename T>
+void launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x);
 template <typename T>
 void launch_rbgs(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
                  const T *b, T *x, int sweeps);
@@ -74,19 +87,27 @@ void launch_sum_partials(hipStream_t st, const double *partial, long nblk, int n
 // precision conversion / accumulation over interior cells
 void launch_d2f(hipStream_t st, const Dims &d, const double *a, float *b);
 void launch_add_f2d(hipStream_t st, const Dims &d, const float *e, double *x);
+void launch_f2d(hipStream_t st, const Dims &d, const float *e, double *x);
 template <typename T>
 void launch_zero(hipStream_t st, const Dims &d, T *a);
 template <typename T>
 void launch_copy(hipStream_t st, const Dims &d, const T *a, T *b);
 
-// multigrid transfers between a fine grid df and a coarse grid dc
+// Multigrid transfers between a fine grid df and a coarse grid dc. Each axis
+// is coarsened independently: a coarse cell covers one or two consecutive
+// fine cells along every axis.
 struct MGTransfer {
-    int cx, cy, cz;               // 1 if the axis is coarsened by 2
-    const int *pi, *pj, *pk;      // fine index -> far coarse neighbour index (or same)
-    const double *wi, *wj, *wk;   // weight of the near coarse cell (1 - weight of far)
-    const double *vol_f;          // unused placeholder
+    const int *ci, *cj, *ck;      // fine index -> coarse (parent) index
+    const int *si, *sj, *sk;      // coarse index -> first fine child (n_coarse + 1 entries)
+    const int *pi, *pj, *pk;      // fine index -> far coarse neighbour index (or the parent)
+    const double *wi, *wj, *wk;   // weight of the parent (1 - weight of the far neighbour)
 };
 
+// bc = restriction of the fluid rows' residual b - A x (one pass, no r)
+template <typename T>
+void launch_resid_restrict(hipStream_t st, const Dims &df, const SysCoef<T> &S, const uint8_t *flag_f, const T *b,
+                           const T *x, const Dims &dc, const MGTransfer &tr, const double *dxyz_f,
+                           const double *dxyz_c, T *bc);
 template <typename T>
 void launch_restrict(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
                      const T *rf, const double *dxf, const double *dyf, const double *dzf, const double *dxc,
@@ -97,6 +118,18 @@ void launch_prolong(hipStream_t st, const Dims &df, const Dims &dc, const MGTran
 template <typename T>
 void launch_coarse_solve(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b, T *x,
                          int sweeps, int singular, const double *dxyz);
+
+// In-place Gauss-Jordan inversion without pivoting of the n x n row-major
+// matrix A (diagonally dominant M-matrix rows); rowp, colp: n doubles of
+// scratch.
+void gauss_jordan_invert(hipStream_t st, double *A, int n, double *rowp, double *colp);
+void launch_d2f_flat(hipStream_t st, const double *a, float *b, long n);
+// x[rows[r]] = sum_c Ainv[r][c] rhs[c], rhs[c] = b[rows[c]]; singular: the
+// volume-weighted mean of b over fluid rows is removed first and the pinned
+// row's right-hand side is zero.
+template <typename T>
+void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, const double *vol,
+                        const uint8_t *flag, int singular, int pinrow, const T *b, T *x);
 
 // One multigrid level as the single-workgroup tail V-cycle sees it. `tr`
 // is the transfer from the next finer level to this one.

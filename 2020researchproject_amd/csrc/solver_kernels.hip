@@ -11,6 +11,10 @@
 // order, then a one-block finalize kernel.
 #include "solver_kernels.h"
 
+#include <algorithm>
+
+#include "rbgs_fused.h"
+
 namespace ibm {
 
 #define C1(a, i) ((a)[(i) + 2])
@@ -44,7 +48,13 @@ __device__ inline void bsum(const double *v, double *partial, long blk) {
 }
 
 static inline dim3 igrid(const Dims &d) { return dim3((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz); }
-static inline long iblocks(const Dims &d) { return (long)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz; }
+// Kernels that reduce walk kRedZ planes per block, so there are kRedZ times
+// fewer block partials for the finalize kernel to sum.
+constexpr int kRedZ = 4;
+static inline dim3 rgrid(const Dims &d) { return dim3((d.nx + 63) / 64, (d.ny + 3) / 4, (d.nz + kRedZ - 1) / kRedZ); }
+static inline long iblocks(const Dims &d) {
+    return (long)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * ((d.nz + kRedZ - 1) / kRedZ);
+}
 
 // Fluid-row stencil: returns diag and the off-diagonal sum sum(c x_nb).
 template <typename T>
@@ -54,20 +64,27 @@ __device__ inline void stencil(const Dims &d, const SysCoef<T> &S, const uint8_t
     off = 0;
     const int id[3] = {i, j, k};
     const long st[3] = {1, (long)d.sy, d.sz};
+    const unsigned m = S.skip ? S.fmask[q] : 0u;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const AxisCoef<T> &A = S.ax[a];
         const int n = A.n, ii = id[a];
         const long qm = (ii == 0 && A.periodic) ? q + (long)(n - 1) * st[a] : q - st[a];
         const long qp = (ii == n - 1 && A.periodic) ? q - (long)(n - 1) * st[a] : q + st[a];
-        T cm = A.cm[ii], cp = A.cp[ii], dm = A.dm[ii], dp = A.dp[ii];
-        if (S.skip) {
-            if ((ii > 0 || A.periodic) && flag[qm] == F_SOLID) { cm = 0; dm = 0; }
-            if ((ii < n - 1 || A.periodic) && flag[qp] == F_SOLID) { cp = 0; dp = 0; }
-        }
-        diag += dm + dp;
-        off += cm * x[qm] + cp * x[qp];
+        axis_term(A, ii, ((m >> (2 * a)) & 1u) != 0, ((m >> (2 * a + 1)) & 1u) != 0, x[qm], x[qp], diag, off);
     }
+}
+
+// diagonal of a fluid row (the stencil with every neighbour value zero)
+template <typename T>
+__device__ inline T stencil_diag(const SysCoef<T> &S, int i, int j, int k, long q) {
+    T diag = S.c0, off = 0;
+    const int id[3] = {i, j, k};
+    const unsigned m = S.skip ? S.fmask[q] : 0u;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        axis_term(S.ax[a], id[a], ((m >> (2 * a)) & 1u) != 0, ((m >> (2 * a + 1)) & 1u) != 0, T(0), T(0), diag, off);
+    return diag;
 }
 
 // ---------------------------------------------------------------------------
@@ -77,28 +94,30 @@ __device__ inline void stencil(const Dims &d, const SysCoef<T> &S, const uint8_t
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_apply(Dims d, SysCoef<T> S, const uint8_t *flag, const T *x, T *y,
                                                const T *w, double *partial) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * kRedZ, k1 = min(k0 + kRedZ, d.nz);
     double acc[2] = {0, 0};
-    if (i < d.nx && j < d.ny) {
-        const long q = cidx(d, i, j, k);
-        const uint8_t f = flag[q];
-        bool have = true;
-        T yv;
-        if (q == S.pin || f == F_SOLID) {
-            yv = x[q];
-        } else if (f == F_FLUID) {
-            T diag, off;
-            stencil(d, S, flag, i, j, k, q, x, diag, off);
-            yv = diag * x[q] - off;
-        } else {
-            have = false;
+    if (i < d.nx && j < d.ny)
+        for (int k = k0; k < k1; k++) {
+            const long q = cidx(d, i, j, k);
+            const uint8_t f = flag[q];
+            bool have = true;
+            T yv;
+            if (q == S.pin || f == F_SOLID) {
+                yv = x[q];
+            } else if (f == F_FLUID) {
+                T diag, off;
+                stencil(d, S, flag, i, j, k, q, x, diag, off);
+                yv = diag * x[q] - off;
+            } else {
+                have = false;
+            }
+            if (have) {
+                y[q] = yv;
+                if (MODE == 1) acc[0] += (double)w[q] * (double)yv;
+                if (MODE == 2) { acc[0] += (double)yv * (double)w[q]; acc[1] += (double)yv * (double)yv; }
+            }
         }
-        if (have) {
-            y[q] = yv;
-            if (MODE == 1) acc[0] = (double)w[q] * (double)yv;
-            if (MODE == 2) { acc[0] = (double)yv * (double)w[q]; acc[1] = (double)yv * (double)yv; }
-        }
-    }
     if constexpr (MODE > 0) bsum<MODE>(acc, partial, blin());
 }
 
@@ -132,13 +151,13 @@ void launch_apply(hipStream_t st, const Dims &d, const SysCoef<T> &S, const Ghos
     const long nb = iblocks(d);
     const int gb = (G.ng + 255) / 256;
     if (mode == 0) {
-        hipLaunchKernelGGL((k_apply<T, 0>), igrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
+        hipLaunchKernelGGL((k_apply<T, 0>), rgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
         if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 0>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
     } else if (mode == 1) {
-        hipLaunchKernelGGL((k_apply<T, 1>), igrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
+        hipLaunchKernelGGL((k_apply<T, 1>), rgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
         if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 1>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
     } else {
-        hipLaunchKernelGGL((k_apply<T, 2>), igrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
+        hipLaunchKernelGGL((k_apply<T, 2>), rgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
         if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 2>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
     }
 }
@@ -155,30 +174,32 @@ __device__ inline double nscale(const NormSpec &ns, const Dims &d, int i, int j,
 template <typename T>
 __global__ void __launch_bounds__(256) k_residual(Dims d, SysCoef<T> S, const uint8_t *flag, const T *b,
                                                   const T *x, T *r, NormSpec ns, double *partial) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * kRedZ, k1 = min(k0 + kRedZ, d.nz);
     double acc[2] = {0, 0};
-    if (i < d.nx && j < d.ny) {
-        const long q = cidx(d, i, j, k);
-        const uint8_t f = flag[q];
-        bool have = true;
-        T rv;
-        if (q == S.pin || f == F_SOLID) {
-            rv = b[q] - x[q];
-        } else if (f == F_FLUID) {
-            T diag, off;
-            stencil(d, S, flag, i, j, k, q, x, diag, off);
-            rv = b[q] - (diag * x[q] - off);
-        } else {
-            have = false;
+    if (i < d.nx && j < d.ny)
+        for (int k = k0; k < k1; k++) {
+            const long q = cidx(d, i, j, k);
+            const uint8_t f = flag[q];
+            bool have = true;
+            T rv;
+            if (q == S.pin || f == F_SOLID) {
+                rv = b[q] - x[q];
+            } else if (f == F_FLUID) {
+                T diag, off;
+                stencil(d, S, flag, i, j, k, q, x, diag, off);
+                rv = b[q] - (diag * x[q] - off);
+            } else {
+                have = false;
+            }
+            if (have) {
+                r[q] = rv;
+                const double rd = (double)rv;
+                acc[0] += rd * rd;
+                const double rs = rd * nscale(ns, d, i, j, k, q);
+                acc[1] += rs * rs;
+            }
         }
-        if (have) {
-            r[q] = rv;
-            const double rd = (double)rv;
-            acc[0] = rd * rd;
-            const double rs = rd * nscale(ns, d, i, j, k, q);
-            acc[1] = rs * rs;
-        }
-    }
     bsum<2>(acc, partial, blin());
 }
 
@@ -208,7 +229,7 @@ template <typename T>
 void launch_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G,
                      const uint8_t *flag, const T *b, const T *x, T *r, const NormSpec &ns, double *partial) {
     const long nb = iblocks(d);
-    hipLaunchKernelGGL((k_residual<T>), igrid(d), dim3(64, 4), 0, st, d, S, flag, b, x, r, ns, partial);
+    hipLaunchKernelGGL((k_residual<T>), rgrid(d), dim3(64, 4), 0, st, d, S, flag, b, x, r, ns, partial);
     const int gb = (G.ng + 255) / 256;
     if (gb) hipLaunchKernelGGL((k_residual_ghost<T>), dim3(gb), dim3(256), 0, st, G, S.is_p, b, x, r, partial, nb);
 }
@@ -256,17 +277,87 @@ void launch_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const Gho
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_rbgs(Dims d, SysCoef<T> S, const uint8_t *flag, const T *b, T *x,
-                                              int color) {
+                                              int color, int zero_in) {
     const int j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
     const int i = 2 * (blockIdx.x * 64 + threadIdx.x) + ((j + k + color) & 1);
     if (i >= d.nx || j >= d.ny) return;
     const long q = cidx(d, i, j, k);
+    if (zero_in) {
+        // the initial x is zero: this lane also clears the other cell of its
+        // pair (2t, 2t + 1), so every interior cell is written once
+        const int ip = i ^ 1;
+        if (ip < d.nx) x[cidx(d, ip, j, k)] = T(0);
+    }
     const uint8_t f = flag[q];
     if (q == S.pin || f == F_SOLID) { x[q] = b[q]; return; }
-    if (f != F_FLUID) return;
+    if (f != F_FLUID) {
+        if (zero_in) x[q] = T(0);
+        return;
+    }
     T diag, off;
-    stencil(d, S, flag, i, j, k, q, x, diag, off);
+    if (zero_in) {
+        diag = stencil_diag(S, i, j, k, q);
+        off = 0;
+    } else {
+        stencil(d, S, flag, i, j, k, q, x, diag, off);
+    }
     x[q] = diag != T(0) ? (b[q] + off) / diag : T(0);
+}
+
+// Residual of the fluid rows restricted to the coarse grid in one pass:
+// bc = (sum over fluid children of r * V_child) / V_coarse, r = b - A x
+// (pinned row: b - x). Equal to the residual kernel followed by the
+// restriction kernel, without storing r.
+template <typename T>
+__global__ void __launch_bounds__(256) k_resid_restrict(Dims df, SysCoef<T> S, const uint8_t *flag_f, const T *b,
+                                                        const T *x, Dims dc, MGTransfer tr, const double *dxf,
+                                                        const double *dyf, const double *dzf, const double *dxc,
+                                                        const double *dyc, const double *dzc, T *bc) {
+    const int I = blockIdx.x * 64 + threadIdx.x, J = blockIdx.y * 4 + threadIdx.y, K = blockIdx.z;
+    if (I >= dc.nx || J >= dc.ny) return;
+    double acc = 0;
+    for (int k = tr.sk[K]; k < tr.sk[K + 1]; k++)
+        for (int j = tr.sj[J]; j < tr.sj[J + 1]; j++)
+            for (int i = tr.si[I]; i < tr.si[I + 1]; i++) {
+                const long q = cidx(df, i, j, k);
+                if (flag_f[q] != F_FLUID) continue;
+                T rv;
+                if (q == S.pin) {
+                    rv = b[q] - x[q];
+                } else {
+                    T diag, off;
+                    stencil(df, S, flag_f, i, j, k, q, x, diag, off);
+                    rv = b[q] - (diag * x[q] - off);
+                }
+                acc += (double)rv * C1(dxf, i) * C1(dyf, j) * C1(dzf, k);
+            }
+    bc[cidx(dc, I, J, K)] = (T)(acc / (C1(dxc, I) * C1(dyc, J) * C1(dzc, K)));
+}
+
+template <typename T>
+void launch_resid_restrict(hipStream_t st, const Dims &df, const SysCoef<T> &S, const uint8_t *flag_f, const T *b,
+                           const T *x, const Dims &dc, const MGTransfer &tr, const double *dxyz_f,
+                           const double *dxyz_c, T *bc) {
+    const double *dxf = dxyz_f, *dyf = dxf + df.nx + 4, *dzf = dyf + df.ny + 4;
+    const double *dxc = dxyz_c, *dyc = dxc + dc.nx + 4, *dzc = dyc + dc.ny + 4;
+    hipLaunchKernelGGL((k_resid_restrict<T>), igrid(dc), dim3(64, 4), 0, st, df, S, flag_f, b, x, dc, tr, dxf, dyf,
+                       dzf, dxc, dyc, dzc, bc);
+}
+
+// planes per workgroup: enough workgroups to fill the chip
+static int rb_kchunk(const Dims &d) {
+    const long tiles = (long)((d.nx + kRbTX - 1) / kRbTX) * ((d.ny + kRbTY - 1) / kRbTY);
+    long kc = (tiles * d.nz + 1023) / 1024;
+    kc = std::max(4L, std::min((long)d.nz, kc));
+    return (int)kc;
+}
+
+template <typename T>
+void launch_rbgs_fused(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
+                       const T *xa, T *xb, int zero_in) {
+    const int kc = rb_kchunk(d);
+    dim3 grid((d.nx + kRbTX - 1) / kRbTX, (d.ny + kRbTY - 1) / kRbTY, (d.nz + kc - 1) / kc);
+    hipLaunchKernelGGL((k_rbgs_fused<T>), grid, dim3(256), 0, st, d, S, flag, b, xa, xb, zero_in, kc);
 }
 
 // Jacobi relaxation of the ghost rows. A ghost stencil may read other ghost
@@ -303,17 +394,27 @@ __global__ void k_ghost_scatter(GhostRows<T> G, T *x) {
 }
 
 template <typename T>
+void launch_rbgs_color(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b, T *x,
+                       int color, int zero_in) {
+    dim3 grid(((d.nx + 1) / 2 + 63) / 64, (d.ny + 3) / 4, d.nz);
+    hipLaunchKernelGGL((k_rbgs<T>), grid, dim3(64, 4), 0, st, d, S, flag, b, x, color, zero_in);
+}
+
+template <typename T>
+void launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x) {
+    const int gb = (G.ng + 255) / 256;
+    if (!gb) return;
+    hipLaunchKernelGGL((k_ghost_relax<T>), dim3(gb), dim3(256), 0, st, G, is_p, b, x);
+    hipLaunchKernelGGL((k_ghost_scatter<T>), dim3(gb), dim3(256), 0, st, G, x);
+}
+
+template <typename T>
 void launch_rbgs(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
                  const T *b, T *x, int sweeps) {
-    dim3 grid(((d.nx + 1) / 2 + 63) / 64, (d.ny + 3) / 4, d.nz);
-    const int gb = (G.ng + 255) / 256;
     for (int s = 0; s < sweeps; s++)
         for (int c = 0; c < 2; c++) {
-            hipLaunchKernelGGL((k_rbgs<T>), grid, dim3(64, 4), 0, st, d, S, flag, b, x, c);
-            if (gb) {
-                hipLaunchKernelGGL((k_ghost_relax<T>), dim3(gb), dim3(256), 0, st, G, S.is_p, b, x);
-                hipLaunchKernelGGL((k_ghost_scatter<T>), dim3(gb), dim3(256), 0, st, G, x);
-            }
+            launch_rbgs_color<T>(st, d, S, flag, b, x, c, 0);
+            launch_ghost_relax<T>(st, G, S.is_p, b, x);
         }
 }
 
@@ -349,19 +450,22 @@ __global__ void __launch_bounds__(256) k_bicg_s(Dims d, const KScal *ks, const T
 template <typename T>
 __global__ void __launch_bounds__(256) k_bicg_xr(Dims d, const KScal *ks, const T *ph, const T *sh, const T *s,
                                                  const T *t, const T *rh, T *x, T *r, NormSpec ns, double *partial) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * kRedZ, k1 = min(k0 + kRedZ, d.nz);
     double acc[3] = {0, 0, 0};
     if (i < d.nx && j < d.ny) {
-        const long q = cidx(d, i, j, k);
         const T alpha = (T)ks->alpha, omega = (T)ks->omega;
-        x[q] = x[q] + alpha * ph[q] + omega * sh[q];
-        const T rv = s[q] - omega * t[q];
-        r[q] = rv;
-        const double rd = (double)rv;
-        acc[0] = rd * rd;
-        const double rs = rd * nscale(ns, d, i, j, k, q);
-        acc[1] = rs * rs;
-        acc[2] = (double)rh[q] * rd;
+        for (int k = k0; k < k1; k++) {
+            const long q = cidx(d, i, j, k);
+            x[q] = x[q] + alpha * ph[q] + omega * sh[q];
+            const T rv = s[q] - omega * t[q];
+            r[q] = rv;
+            const double rd = (double)rv;
+            acc[0] += rd * rd;
+            const double rs = rd * nscale(ns, d, i, j, k, q);
+            acc[1] += rs * rs;
+            acc[2] += (double)rh[q] * rd;
+        }
     }
     bsum<3>(acc, partial, blin());
 }
@@ -383,24 +487,48 @@ void launch_bicg_s(hipStream_t st, const Dims &d, const KScal *ks, const T *r, c
 template <typename T>
 void launch_bicg_xr(hipStream_t st, const Dims &d, const KScal *ks, const T *ph, const T *sh, const T *s,
                     const T *t, const T *rh, T *x, T *r, const NormSpec &ns, double *partial) {
-    hipLaunchKernelGGL((k_bicg_xr<T>), igrid(d), dim3(64, 4), 0, st, d, ks, ph, sh, s, t, rh, x, r, ns, partial);
+    hipLaunchKernelGGL((k_bicg_xr<T>), rgrid(d), dim3(64, 4), 0, st, d, ks, ph, sh, s, t, rh, x, r, ns, partial);
 }
 
-// one block of 256 threads; thread t sums partials t, t+256, ... in order,
-// then a fixed-shape tree: bitwise reproducible for a given grid.
-__global__ void __launch_bounds__(256) k_finalize(int op, const double *partial, long nblk, int nv, KScal *ks) {
-    __shared__ double sh[4][256];
+// One block of kFinThreads threads sums the block partials: thread t adds
+// partials t, t + kFinThreads, ... in order, then a fixed-shape tree, so the
+// result is bitwise reproducible for a given grid.
+constexpr int kFinThreads = 1024;
+
+__device__ inline void sum_columns(const double *partial, long nblk, int nv, int reduce_max,
+                                   double (&sh)[4][kFinThreads]) {
     const int t = threadIdx.x;
     double acc[4] = {0, 0, 0, 0};
-    for (long b = t; b < nblk; b += 256)
-        for (int a = 0; a < nv; a++) acc[a] += partial[b * nv + a];
+    long b = t;
+    // four rows in flight per thread
+    for (; b + 3 * kFinThreads < nblk; b += 4 * kFinThreads) {
+        double v[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            for (int a = 0; a < nv; a++) v[u][a] = partial[(b + u * kFinThreads) * nv + a];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            for (int a = 0; a < nv; a++) acc[a] = reduce_max ? fmax(acc[a], v[u][a]) : acc[a] + v[u][a];
+    }
+    for (; b < nblk; b += kFinThreads)
+        for (int a = 0; a < nv; a++) {
+            const double v = partial[b * nv + a];
+            acc[a] = reduce_max ? fmax(acc[a], v) : acc[a] + v;
+        }
     for (int a = 0; a < nv; a++) sh[a][t] = acc[a];
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = kFinThreads / 2; s > 0; s >>= 1) {
         if (t < s)
-            for (int a = 0; a < nv; a++) sh[a][t] += sh[a][t + s];
+            for (int a = 0; a < nv; a++) sh[a][t] = reduce_max ? fmax(sh[a][t], sh[a][t + s]) : sh[a][t] + sh[a][t + s];
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_finalize(int op, const double *partial, long nblk, int nv,
+                                                          KScal *ks) {
+    __shared__ double sh[4][kFinThreads];
+    const int t = threadIdx.x;
+    sum_columns(partial, nblk, nv, 0, sh);
     if (t != 0) return;
     const double s0 = sh[0][0], s1 = nv > 1 ? sh[1][0] : 0, s2 = nv > 2 ? sh[2][0] : 0;
     switch (op) {
@@ -437,29 +565,20 @@ __global__ void __launch_bounds__(256) k_finalize(int op, const double *partial,
 }
 
 // sums nv partial columns into out[0..nv) (same fixed order as k_finalize)
-__global__ void __launch_bounds__(256) k_sum_partials(const double *partial, long nblk, int nv, double *out, int reduce_max) {
-    __shared__ double sh[4][256];
-    const int t = threadIdx.x;
-    double acc[4] = {0, 0, 0, 0};
-    for (long b = t; b < nblk; b += 256)
-        for (int a = 0; a < nv; a++) acc[a] = reduce_max ? fmax(acc[a], partial[b * nv + a]) : acc[a] + partial[b * nv + a];
-    for (int a = 0; a < nv; a++) sh[a][t] = acc[a];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s)
-            for (int a = 0; a < nv; a++) sh[a][t] = reduce_max ? fmax(sh[a][t], sh[a][t + s]) : sh[a][t] + sh[a][t + s];
-        __syncthreads();
-    }
-    if (t == 0)
+__global__ void __launch_bounds__(kFinThreads) k_sum_partials(const double *partial, long nblk, int nv, double *out,
+                                                              int reduce_max) {
+    __shared__ double sh[4][kFinThreads];
+    sum_columns(partial, nblk, nv, reduce_max, sh);
+    if (threadIdx.x == 0)
         for (int a = 0; a < nv; a++) out[a] = sh[a][0];
 }
 
 void launch_sum_partials(hipStream_t st, const double *partial, long nblk, int nv, double *out, int reduce_max) {
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partial, nblk, nv, out, reduce_max);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kFinThreads), 0, st, partial, nblk, nv, out, reduce_max);
 }
 
 void launch_finalize(hipStream_t st, int op, const double *partial, long nblk, int nv, KScal *ks) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, op, partial, nblk, nv, ks);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, st, op, partial, nblk, nv, ks);
 }
 
 // ---------------------------------------------------------------------------
@@ -476,6 +595,12 @@ __global__ void __launch_bounds__(256) k_add_f2d(Dims d, const float *e, double 
     if (i >= d.nx || j >= d.ny) return;
     const long q = cidx(d, i, j, k);
     x[q] += (double)e[q];
+}
+__global__ void __launch_bounds__(256) k_f2d(Dims d, const float *e, double *x) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    x[q] = (double)e[q];
 }
 template <typename T>
 __global__ void __launch_bounds__(256) k_zero(Dims d, T *a) {
@@ -495,6 +620,9 @@ void launch_d2f(hipStream_t st, const Dims &d, const double *a, float *b) {
 }
 void launch_add_f2d(hipStream_t st, const Dims &d, const float *e, double *x) {
     hipLaunchKernelGGL(k_add_f2d, igrid(d), dim3(64, 4), 0, st, d, e, x);
+}
+void launch_f2d(hipStream_t st, const Dims &d, const float *e, double *x) {
+    hipLaunchKernelGGL(k_f2d, igrid(d), dim3(64, 4), 0, st, d, e, x);
 }
 template <typename T>
 void launch_zero(hipStream_t st, const Dims &d, T *a) {
@@ -517,11 +645,9 @@ __global__ void __launch_bounds__(256) k_restrict(Dims df, Dims dc, MGTransfer t
     const int I = blockIdx.x * 64 + threadIdx.x, J = blockIdx.y * 4 + threadIdx.y, K = blockIdx.z;
     if (I >= dc.nx || J >= dc.ny) return;
     double acc = 0;
-    const int i0 = tr.cx ? 2 * I : I, j0 = tr.cy ? 2 * J : J, k0 = tr.cz ? 2 * K : K;
-    for (int c = 0; c <= tr.cz; c++)
-        for (int b = 0; b <= tr.cy; b++)
-            for (int a = 0; a <= tr.cx; a++) {
-                const int i = i0 + a, j = j0 + b, k = k0 + c;
+    for (int k = tr.sk[K]; k < tr.sk[K + 1]; k++)
+        for (int j = tr.sj[J]; j < tr.sj[J + 1]; j++)
+            for (int i = tr.si[I]; i < tr.si[I + 1]; i++) {
                 const long q = cidx(df, i, j, k);
                 if (flag_f[q] != F_FLUID) continue;
                 acc += (double)rf[q] * C1(dxf, i) * C1(dyf, j) * C1(dzf, k);
@@ -537,7 +663,7 @@ __global__ void __launch_bounds__(256) k_prolong(Dims df, Dims dc, MGTransfer tr
     if (i >= df.nx || j >= df.ny) return;
     const long q = cidx(df, i, j, k);
     if (flag_f[q] != F_FLUID) return;
-    const int I = tr.cx ? i >> 1 : i, J = tr.cy ? j >> 1 : j, K = tr.cz ? k >> 1 : k;
+    const int I = tr.ci[i], J = tr.cj[j], K = tr.ck[k];
     const int Is[2] = {I, tr.pi[i]}, Js[2] = {J, tr.pj[j]}, Ks[2] = {K, tr.pk[k]};
     const double wi[2] = {tr.wi[i], 1 - tr.wi[i]}, wj[2] = {tr.wj[j], 1 - tr.wj[j]}, wk[2] = {tr.wk[k], 1 - tr.wk[k]};
     const long q0 = cidx(dc, I, J, K);
@@ -720,11 +846,10 @@ __global__ void __launch_bounds__(1024) k_tail_vcycle(const LevelDev<T> *lv, int
             int I, J, K;
             decode(c, dc, I, J, K);
             double acc = 0;
-            const int i0 = Cl.tr.cx ? 2 * I : I, j0 = Cl.tr.cy ? 2 * J : J, k0 = Cl.tr.cz ? 2 * K : K;
-            for (int cz = 0; cz <= Cl.tr.cz; cz++)
-                for (int cy = 0; cy <= Cl.tr.cy; cy++)
-                    for (int cx = 0; cx <= Cl.tr.cx; cx++) {
-                        const int i = i0 + cx, j = j0 + cy, k = k0 + cz;
+            const MGTransfer &tr = Cl.tr;
+            for (int k = tr.sk[K]; k < tr.sk[K + 1]; k++)
+                for (int j = tr.sj[J]; j < tr.sj[J + 1]; j++)
+                    for (int i = tr.si[I]; i < tr.si[I + 1]; i++) {
                         const long q = cidx(d, i, j, k);
                         if (L.flag[q] != F_FLUID) continue;
                         acc += (double)L.r[q] * L.dx[i + 2] * L.dy[j + 2] * L.dz[k + 2];
@@ -835,7 +960,7 @@ __global__ void __launch_bounds__(1024) k_tail_vcycle(const LevelDev<T> *lv, int
             decode(c, d, i, j, k);
             const long q = cidx(d, i, j, k);
             if (L.flag[q] != F_FLUID) continue;
-            const int I = tr.cx ? i >> 1 : i, J = tr.cy ? j >> 1 : j, K = tr.cz ? k >> 1 : k;
+            const int I = tr.ci[i], J = tr.cj[j], K = tr.ck[k];
             const int Is[2] = {I, tr.pi[i]}, Js[2] = {J, tr.pj[j]}, Ks[2] = {K, tr.pk[k]};
             const double wi[2] = {tr.wi[i], 1 - tr.wi[i]}, wj[2] = {tr.wj[j], 1 - tr.wj[j]},
                          wk[2] = {tr.wk[k], 1 - tr.wk[k]};
@@ -863,9 +988,118 @@ void launch_tail_vcycle(hipStream_t st, const LevelDev<T> *lv, int l0, int nl, i
                        singular);
 }
 
+// ---------------------------------------------------------------------------
+// exact coarsest-level solve through a precomputed dense inverse
+// ---------------------------------------------------------------------------
+__global__ void k_gj_prep(const double *A, int n, int p, double *rowp, double *colp) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const double piv = A[(long)p * n + p];
+    rowp[t] = (t == p ? 1.0 : A[(long)p * n + t]) / piv;
+    colp[t] = t == p ? 0.0 : A[(long)t * n + p];
+}
+
+__global__ void k_gj_update(double *A, int n, int p, const double *rowp, const double *colp) {
+    const long e = blockIdx.x * 256L + threadIdx.x;
+    if (e >= (long)n * n) return;
+    const int r = (int)(e / n), c = (int)(e % n);
+    if (r == p) A[e] = rowp[c];
+    else A[e] = (c == p ? 0.0 : A[e]) - colp[r] * rowp[c];
+}
+
+void gauss_jordan_invert(hipStream_t st, double *A, int n, double *rowp, double *colp) {
+    const int gp = (n + 255) / 256;
+    const long gu = ((long)n * n + 255) / 256;
+    for (int p = 0; p < n; p++) {
+        hipLaunchKernelGGL(k_gj_prep, dim3(gp), dim3(256), 0, st, A, n, p, rowp, colp);
+        hipLaunchKernelGGL(k_gj_update, dim3(gu), dim3(256), 0, st, A, n, p, rowp, colp);
+    }
+}
+
+__global__ void k_d2f_flat(const double *a, float *b, long n) {
+    const long e = blockIdx.x * 256L + threadIdx.x;
+    if (e < n) b[e] = (float)a[e];
+}
+
+void launch_d2f_flat(hipStream_t st, const double *a, float *b, long n) {
+    hipLaunchKernelGGL(k_d2f_flat, dim3((n + 255) / 256), dim3(256), 0, st, a, b, n);
+}
+
+constexpr int kDenseMaxRows = 4096;  // right-hand side staged in LDS
+
+// One wavefront per row of the inverse; four independent partial sums per
+// lane keep several row loads in flight.
+template <typename T>
+__global__ void __launch_bounds__(256) k_dense_solve(const T *Ainv, int n, const long *rows, const double *vol,
+                                                     const uint8_t *flag, int singular, int pinrow, const T *b,
+                                                     T *x) {
+    __shared__ double rhs[kDenseMaxRows];
+    __shared__ double red[2][4];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    double bmean = 0;
+    if (singular) {
+        double sv = 0, sb = 0;
+        for (int c = t; c < n; c += 256) {
+            const long q = rows[c];
+            if (flag[q] != F_FLUID) continue;
+            sv += vol[c];
+            sb += vol[c] * (double)b[q];
+        }
+        for (int o = 32; o > 0; o >>= 1) { sv += __shfl_down(sv, o, 64); sb += __shfl_down(sb, o, 64); }
+        if (lane == 0) { red[0][wave] = sv; red[1][wave] = sb; }
+        __syncthreads();
+        const double V = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        const double B = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        bmean = V > 0 ? B / V : 0;
+    }
+    for (int c = t; c < n; c += 256) {
+        const long q = rows[c];
+        double v = (double)b[q];
+        if (singular) {
+            if (c == pinrow) v = 0;
+            else if (flag[q] == F_FLUID) v -= bmean;
+        }
+        rhs[c] = v;
+    }
+    __syncthreads();
+    const int r = blockIdx.x * 4 + wave;
+    if (r >= n) return;
+    const T *row = Ainv + (long)r * n;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int c = lane;
+    for (; c + 192 < n; c += 256) {
+        a0 += (double)row[c] * rhs[c];
+        a1 += (double)row[c + 64] * rhs[c + 64];
+        a2 += (double)row[c + 128] * rhs[c + 128];
+        a3 += (double)row[c + 192] * rhs[c + 192];
+    }
+    for (; c < n; c += 64) a0 += (double)row[c] * rhs[c];
+    double acc = (a0 + a1) + (a2 + a3);
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if (lane == 0) x[rows[r]] = (T)acc;
+}
+
+template <typename T>
+void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, const double *vol,
+                        const uint8_t *flag, int singular, int pinrow, const T *b, T *x) {
+    if (n > kDenseMaxRows) abort();
+    hipLaunchKernelGGL((k_dense_solve<T>), dim3((n + 3) / 4), dim3(256), 0, st, Ainv, n, rows, vol, flag, singular,
+                       pinrow, b, x);
+}
+
 // explicit instantiations
 #define INST(T)                                                                                                   \
     template void launch_tail_vcycle<T>(hipStream_t, const LevelDev<T> *, int, int, int, int, int, int);            \
+    template void launch_dense_solve<T>(hipStream_t, const T *, int, const long *, const double *, const uint8_t *, \
+                                        int, int, const T *, T *);                                                \
+    template void launch_rbgs_fused<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *, const T *,  \
+                                       const T *, T *, int);                                                      \
+    template void launch_rbgs_color<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *, const T *, T *, \
+                                       int, int);                                                                 \
+    template void launch_resid_restrict<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *,         \
+                                           const T *, const T *, const Dims &, const MGTransfer &, const double *, \
+                                           const double *, T *);                                                  \
+    template void launch_ghost_relax<T>(hipStream_t, const GhostRows<T> &, int, const T *, T *);                   \
     template void launch_apply<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,             \
                                   const uint8_t *, const T *, T *, int, const T *, double *);                      \
     template void launch_residual<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,          \

@@ -16,6 +16,8 @@
 
 #include "solver.h"
 
+#include <type_traits>
+
 namespace ibm {
 
 #define H1(a, i) ((a)[(i) + 2])
@@ -163,9 +165,9 @@ static void read_ks(IBMSolver *s) {
 // ---------------------------------------------------------------------------
 // multigrid hierarchy of the pressure system
 // ---------------------------------------------------------------------------
-// coarsening stops at this many cells; levels at or below kTailCells run in
-// one workgroup (launch latency, not bandwidth, bounds them)
-static const long kCoarsestCells = 16, kTailCells = 4096;
+// coarsening stops at the first level below the fine grid with at most this
+// many cells; that level is solved exactly through a dense inverse
+static const long kDenseCells = 3000;
 
 static void level_geometry(IBMSolver *s, MGLevel &L, const std::vector<double> &xf, const std::vector<double> &yf,
                            const std::vector<double> &zf) {
@@ -201,6 +203,65 @@ static void kcoef(const IBMSolver *s, const std::vector<double> &D, const std::v
     }
 }
 
+// Dense matrix of a coarse level's pressure operator (rows as the matrix-free
+// stencil with solid faces skipped; inactive cells and, for a singular
+// problem, the first fluid cell get identity rows), inverted on the device.
+static void build_dense(IBMSolver *s, MGLevel &L) {
+    const Dims &d = L.d;
+    const int n = d.nx * d.ny * d.nz;
+    const SysData &S = L.sys;
+    std::vector<double> A((size_t)n * n, 0.0), vol(n);
+    std::vector<long> rows(n);
+    int pin = -1;
+    for (int k = 0, c = 0; k < d.nz; k++)
+        for (int j = 0; j < d.ny; j++)
+            for (int i = 0; i < d.nx; i++, c++) {
+                rows[c] = cidx(d, i, j, k);
+                vol[c] = H1(L.dx, i) * H1(L.dy, j) * H1(L.dz, k);
+                if (s->p_singular && pin < 0 && L.hflag[rows[c]] == F_FLUID) pin = c;
+            }
+    const int nn[3] = {d.nx, d.ny, d.nz};
+    const int stride[3] = {1, d.nx, d.nx * d.ny};
+    for (int c = 0; c < n; c++) {
+        const long q = rows[c];
+        double *row = &A[(size_t)c * n];
+        if (L.hflag[q] != F_FLUID || c == pin) { row[c] = 1; continue; }
+        const int id[3] = {c % d.nx, (c / d.nx) % d.ny, c / (d.nx * d.ny)};
+        double diag = S.S64.c0;
+        for (int a = 0; a < 3; a++) {
+            const int ii = id[a], na = nn[a];
+            const bool per = S.S64.ax[a].periodic != 0;
+            double cm = S.h[a][0][ii], cp = S.h[a][1][ii], dm = S.h[a][2][ii], dp = S.h[a][3][ii];
+            const bool hm = ii > 0 || per, hp = ii < na - 1 || per;
+            const int cmi = c + ((ii == 0 ? na - 1 : -1)) * stride[a];
+            const int cpi = c + ((ii == na - 1 ? -(na - 1) : 1)) * stride[a];
+            if (hm && L.hflag[rows[cmi]] == F_SOLID) { cm = 0; dm = 0; }
+            if (hp && L.hflag[rows[cpi]] == F_SOLID) { cp = 0; dp = 0; }
+            diag += dm + dp;
+            if (hm) row[cmi] -= cm;
+            if (hp) row[cpi] -= cp;
+        }
+        row[c] += diag;
+    }
+    L.dn = n;
+    L.dpin = pin;
+    L.dinv64.upload(A);
+    L.drow.upload(rows);
+    L.dvol.upload(vol);
+    DBuf<double> rowp, colp;
+    rowp.alloc(n);
+    colp.alloc(n);
+    gauss_jordan_invert(s->stream, L.dinv64.p, n, rowp.p, colp.p);
+    if (s->precision != IBM3D_PREC_FP64) {
+        L.dinv32.alloc((size_t)n * n);
+        launch_d2f_flat(s->stream, L.dinv64.p, L.dinv32.p, (long)n * n);
+    }
+    HIP_CHECK(hipMemcpyAsync(A.data(), L.dinv64.p, A.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    for (double v : A)
+        if (!std::isfinite(v)) fatal("coarsest multigrid level: singular dense operator");
+}
+
 void build_mg(IBMSolver *s) {
     s->mg.clear();
     s->mg.emplace_back();
@@ -218,22 +279,42 @@ void build_mg(IBMSolver *s) {
     }
     std::vector<double> xf = s->xf, yf = s->yf, zf = s->zf;
     const int lo[3] = {3, 2, 4}, hi[3] = {1, 0, 5};
+    double h0 = 1e300;
+    for (const std::vector<double> *f : {&xf, &yf, &zf})
+        for (size_t i = 0; i + 1 < f->size(); i++) h0 = std::min(h0, (*f)[i + 1] - (*f)[i]);
     while (true) {
         const MGLevel &P = s->mg.back();
         const int n[3] = {P.d.nx, P.d.ny, P.d.nz};
-        int c[3];
-        for (int a = 0; a < 3; a++) c[a] = (n[a] % 2 == 0 && n[a] >= 2) ? 1 : 0;
-        if ((!c[0] && !c[1] && !c[2]) || (long)n[0] * n[1] * n[2] <= kCoarsestCells) break;
-        // coarse faces: every other face on coarsened axes
-        auto coarsen = [](const std::vector<double> &f, int cf) {
-            if (!cf) return f;
-            std::vector<double> g;
-            for (size_t q = 0; q < f.size(); q += 2) g.push_back(f[q]);
-            return g;
-        };
-        xf = coarsen(xf, c[0]); yf = coarsen(yf, c[1]); zf = coarsen(zf, c[2]);
+        if (s->mg.size() > 1 && (long)n[0] * n[1] * n[2] <= kDenseCells) break;
+        // Spacing-aware coarsening: along each axis, two neighbouring cells
+        // merge when their combined width is at most 2 * gamma * h_l, where
+        // h_l = 2^l * (smallest fine-grid width) is the target width of level
+        // l. Where one axis is much finer than the others only that axis
+        // coarsens (semi-coarsening in the strongly coupled direction), so the
+        // point smoother stays effective on stretched grids; gamma <= 0
+        // selects plain full coarsening of even-sized axes.
+        std::vector<double> *F[3] = {&xf, &yf, &zf};
+        const double thr = 2 * s->mg_gamma * h0 * std::ldexp(1.0, (int)s->mg.size() - 1) * (1 + 1e-9);
+        std::vector<int> parent[3], start[3];
+        bool merged = false;
+        for (int a = 0; a < 3; a++) {
+            const std::vector<double> &f = *F[a];
+            const bool full = s->mg_gamma <= 0 && n[a] % 2 == 0 && n[a] >= 2;
+            std::vector<double> g{f[0]};
+            parent[a].assign(n[a], 0);
+            for (int i = 0; i < n[a];) {
+                const bool pair = i + 1 < n[a] && (full || (s->mg_gamma > 0 && f[i + 2] - f[i] <= thr));
+                start[a].push_back(i);
+                parent[a][i] = (int)start[a].size() - 1;
+                if (pair) { parent[a][i + 1] = parent[a][i]; merged = true; }
+                i += pair ? 2 : 1;
+                g.push_back(f[i]);
+            }
+            start[a].push_back(n[a]);
+            *F[a] = g;
+        }
+        if (!merged) break;
         MGLevel L;
-        L.cx = c[0]; L.cy = c[1]; L.cz = c[2];
         const int m[3] = {(int)xf.size() - 1, (int)yf.size() - 1, (int)zf.size() - 1};
         L.d = make_dims(m[0], m[1], m[2]);
         level_geometry(s, L, xf, yf, zf);
@@ -243,12 +324,10 @@ void build_mg(IBMSolver *s) {
             for (int J = 0; J < m[1]; J++)
                 for (int I = 0; I < m[0]; I++) {
                     bool act = false;
-                    for (int kk = 0; kk <= c[2]; kk++)
-                        for (int jj = 0; jj <= c[1]; jj++)
-                            for (int ii = 0; ii <= c[0]; ii++) {
-                                const int i = c[0] ? 2 * I + ii : I, j = c[1] ? 2 * J + jj : J, k = c[2] ? 2 * K + kk : K;
+                    for (int k = start[2][K]; k < start[2][K + 1]; k++)
+                        for (int j = start[1][J]; j < start[1][J + 1]; j++)
+                            for (int i = start[0][I]; i < start[0][I + 1]; i++)
                                 if (P.hflag[cidx(P.d, i, j, k)] == F_FLUID) act = true;
-                            }
                     L.hflag[cidx(L.d, I, J, K)] = act ? F_FLUID : F_SOLID;
                 }
         L.flag.upload(L.hflag);
@@ -269,49 +348,83 @@ void build_mg(IBMSolver *s) {
             for (int h = 0; h < 2; h++) bct[a][h] = s->bc[face_bc(a, h)].type;
         build_system(s, D, C, Km, Kp, m, bct, 3, L.sys);
         L.sys.S64.skip = L.sys.S32.skip = 1;
-        // trilinear prolongation data over the finer grid's indices
+        // faces to inactive (solid) neighbours, per cell
+        {
+            std::vector<uint8_t> fm(L.d.ncell, 0);
+            for (int K = 0; K < m[2]; K++)
+                for (int J = 0; J < m[1]; J++)
+                    for (int I = 0; I < m[0]; I++) {
+                        const int id[3] = {I, J, K};
+                        uint8_t bits = 0;
+                        for (int a = 0; a < 3; a++) {
+                            const bool per = L.sys.S64.ax[a].periodic != 0;
+                            int lo_[3] = {I, J, K}, hi_[3] = {I, J, K};
+                            lo_[a] = id[a] == 0 ? (per ? m[a] - 1 : -1) : id[a] - 1;
+                            hi_[a] = id[a] == m[a] - 1 ? (per ? 0 : -1) : id[a] + 1;
+                            if (lo_[a] >= 0 && L.hflag[cidx(L.d, lo_[0], lo_[1], lo_[2])] == F_SOLID) bits |= 1u << (2 * a);
+                            if (hi_[a] >= 0 && L.hflag[cidx(L.d, hi_[0], hi_[1], hi_[2])] == F_SOLID)
+                                bits |= 1u << (2 * a + 1);
+                        }
+                        fm[cidx(L.d, I, J, K)] = bits;
+                    }
+            L.fmask.upload(fm);
+            L.sys.S64.fmask = L.fmask.p;
+            L.sys.S32.fmask = L.fmask.p;
+        }
+        // trilinear prolongation over the finer grid's cell centres: the
+        // parent and the coarse neighbour on the far side of the fine centre
         const std::vector<double> *cf[3] = {&P.xc, &P.yc, &P.zc}, *cc[3] = {&L.xc, &L.yc, &L.zc};
-        DBuf<int> *pidx[3] = {&L.pi, &L.pj, &L.pk};
+        DBuf<int> *pidx[3] = {&L.pi, &L.pj, &L.pk}, *cpar[3] = {&L.ci, &L.cj, &L.ck}, *cst[3] = {&L.si, &L.sj, &L.sk};
         DBuf<double> *pw[3] = {&L.wi, &L.wj, &L.wk};
         for (int a = 0; a < 3; a++) {
             std::vector<int> far(n[a]);
             std::vector<double> w(n[a]);
             const bool per = s->bc[lo[a]].type == BC_ALL_PERIODIC && s->bc[hi[a]].type == BC_ALL_PERIODIC;
             for (int i = 0; i < n[a]; i++) {
-                if (!c[a]) { far[i] = i; w[i] = 1; continue; }
-                const int I = i >> 1;
-                int J = (i & 1) ? I + 1 : I - 1;
+                const int I = parent[a][i];
                 const double xf_ = H1((*cf[a]), i), Xn = H1((*cc[a]), I);
+                if (start[a][I + 1] - start[a][I] == 1) { far[i] = I; w[i] = 1; continue; }
+                const int J = xf_ > Xn ? I + 1 : I - 1;
+                const double Xf = H1((*cc[a]), J);  // outer-layer centre beyond the ends
+                const double wn = (Xf - xf_) / (Xf - Xn);
                 if (J < 0 || J >= m[a]) {
                     if (per) {
-                        const double Xf = H1((*cc[a]), J);  // ghost centre of the periodic image
-                        w[i] = (Xf - xf_) / (Xf - Xn);
+                        w[i] = wn;
                         far[i] = (J + m[a]) % m[a];
                     } else {
                         const IBMSolverBCType t = s->bc[J < 0 ? lo[a] : hi[a]].type;
                         const bool dir = t == BC_PRESSURE || t == BC_VELOCITY_PERIODIC;
-                        const double Xf = H1((*cc[a]), J);
-                        const double wn = (Xf - xf_) / (Xf - Xn);
                         w[i] = dir ? wn - (1 - wn) : 1.0;  // Dirichlet: odd mirror, Neumann: even
                         far[i] = I;
                     }
                 } else {
-                    const double Xf = H1((*cc[a]), J);
-                    w[i] = (Xf - xf_) / (Xf - Xn);
+                    w[i] = wn;
                     far[i] = J;
                 }
             }
             pidx[a]->upload(far);
             pw[a]->upload(w);
+            cpar[a]->upload(parent[a]);
+            cst[a]->upload(start[a]);
         }
-        L.tr = {c[0], c[1], c[2], L.pi.p, L.pj.p, L.pk.p, L.wi.p, L.wj.p, L.wk.p, nullptr};
-        L.x64.alloc(L.d.ncell); L.b64.alloc(L.d.ncell); L.r64.alloc(L.d.ncell);
-        if (s->precision == IBM3D_PREC_MIXED) { L.x32.alloc(L.d.ncell); L.b32.alloc(L.d.ncell); L.r32.alloc(L.d.ncell); }
+        L.tr = {L.ci.p, L.cj.p, L.ck.p, L.si.p, L.sj.p, L.sk.p, L.pi.p, L.pj.p, L.pk.p, L.wi.p, L.wj.p, L.wk.p};
+        L.x64.alloc(L.d.ncell); L.b64.alloc(L.d.ncell); L.r64.alloc(L.d.ncell); L.xa64.alloc(L.d.ncell);
+        if (s->precision != IBM3D_PREC_FP64) {
+            L.x32.alloc(L.d.ncell); L.b32.alloc(L.d.ncell); L.r32.alloc(L.d.ncell); L.xa32.alloc(L.d.ncell);
+        }
         s->mg.push_back(std::move(L));
+    }
+    if (s->mg.size() > 1) {
+        MGLevel &Lc = s->mg.back();
+        if ((long)Lc.d.nx * Lc.d.ny * Lc.d.nz <= kDenseCells) build_dense(s, Lc);
     }
     // fine-level residual scratch
     s->mg[0].r64.alloc(s->d.ncell);
-    if (s->precision == IBM3D_PREC_MIXED) s->mg[0].r32.alloc(s->d.ncell);
+    s->mg[0].xa64.alloc(s->d.ncell);
+    if (s->precision != IBM3D_PREC_FP64) {
+        s->mg[0].r32.alloc(s->d.ncell);
+        s->mg[0].xa32.alloc(s->d.ncell);
+    }
     // level descriptors of the single-workgroup tail V-cycle (levels >= 1)
     const int nl = (int)s->mg.size();
     std::vector<LevelDev<double>> l64(nl);
@@ -326,6 +439,12 @@ void build_mg(IBMSolver *s) {
     s->mg_lv32.upload(l32);
 }
 
+template <typename T> static const T *dinv(const MGLevel &L);
+template <> const double *dinv<double>(const MGLevel &L) { return L.dinv64.p; }
+template <> const float *dinv<float>(const MGLevel &L) { return L.dinv32.p; }
+template <typename T> static DBuf<T> &lev_xa(MGLevel &L);
+template <> DBuf<double> &lev_xa<double>(MGLevel &L) { return L.xa64; }
+template <> DBuf<float> &lev_xa<float>(MGLevel &L) { return L.xa32; }
 template <typename T> static DBuf<T> &lev_r(MGLevel &L);
 template <> DBuf<double> &lev_r<double>(MGLevel &L) { return L.r64; }
 template <> DBuf<float> &lev_r<float>(MGLevel &L) { return L.r32; }
@@ -345,12 +464,48 @@ template <typename T> static DBuf<T> *kvec(Krylov &K);
 template <> DBuf<double> *kvec<double>(Krylov &K) { return K.v64; }
 template <> DBuf<float> *kvec<float>(Krylov &K) { return K.v32; }
 
-static const int kPre = 2, kPost = 2, kCoarse = 40;
+static const int kCoarse = 40;
 
 // x = V-cycle approximation of A^-1 b on level l (x0 = 0)
 template <typename T> static const LevelDev<T> *lvdev(IBMSolver *s);
 template <> const LevelDev<double> *lvdev<double>(IBMSolver *s) { return s->mg_lv64.p; }
 template <> const LevelDev<float> *lvdev<float>(IBMSolver *s) { return s->mg_lv32.p; }
+
+// Red-black Gauss-Seidel sweeps on x (zero_init: the initial x is zero and
+// is not read). Smoother 0: one fused out-of-place launch per sweep,
+// alternating between x and xalt, then a Jacobi pass over the ghost rows;
+// algorithmic bytes per sweep: x and b read, x written, one flag byte per
+// cell. Smoother 1: one launch per colour (x read once, b read and x
+// written on half of the cells), ghost rows relaxed after each colour.
+template <typename T>
+static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
+                   const T *b, T *x, T *xalt, int sweeps, bool zero_init, bool fine) {
+    const double n = (double)d.nx * d.ny * d.nz;
+    if (sweeps == 0) {
+        if (zero_init) TIMED("mg_zero", n * sizeof(T), launch_zero<T>(s->stream, d, x));
+        return;
+    }
+    if (s->mg_smoother == 1) {
+        for (int sw = 0; sw < sweeps; sw++)
+            for (int c = 0; c < 2; c++) {
+                // the first colour from a zero start writes every cell
+                const bool z = zero_init && sw == 0 && c == 0;
+                TIMED(fine ? "rbgs_fine" : "rbgs", z ? n * (1.5 * sizeof(T) + 1) : n * (2 * sizeof(T) + 1),
+                      launch_rbgs_color<T>(s->stream, d, S, flag, b, x, c, z ? 1 : 0));
+                if (G.ng) TIMED("ghost_relax", 0, launch_ghost_relax<T>(s->stream, G, S.is_p, b, x));
+            }
+        return;
+    }
+    T *cur = x, *nxt = xalt;
+    for (int sw = 0; sw < sweeps; sw++) {
+        const bool z = zero_init && sw == 0;
+        TIMED(fine ? "rbgs_fused_fine" : "rbgs_fused", n * ((z ? 2 : 3) * sizeof(T) + 1),
+              launch_rbgs_fused<T>(s->stream, d, S, flag, b, cur, nxt, z ? 1 : 0));
+        if (G.ng) TIMED("ghost_relax", 0, launch_ghost_relax<T>(s->stream, G, S.is_p, b, nxt));
+        std::swap(cur, nxt);
+    }
+    if (cur != x) TIMED("mg_copy", 2 * n * sizeof(T), launch_copy<T>(s->stream, d, cur, x));
+}
 
 template <typename T>
 static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
@@ -360,10 +515,10 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
     const GhostRows<T> G0 = fine ? grows<T>(s->gp) : GhostRows<T>{0, nullptr, nullptr, nullptr, nullptr, nullptr};
     const uint8_t *flag = fine ? s->flag.p : L.flag.p;
     const double cell_b = (double)L.d.nx * L.d.ny * L.d.nz * sizeof(T);
-    if (l >= 1 && (long)L.d.nx * L.d.ny * L.d.nz <= kTailCells) {
-        // b of level l is in place (written by the restriction)
-        TIMED("mg_tail", 0, launch_tail_vcycle<T>(s->stream, lvdev<T>(s), l, (int)s->mg.size(), kPre, kPost, kCoarse,
-                                                  s->p_singular ? 1 : 0));
+    if (l >= 1 && L.dn > 0) {
+        TIMED("mg_dense", (double)L.dn * L.dn * sizeof(T),
+              launch_dense_solve<T>(s->stream, dinv<T>(L), L.dn, L.drow.p, L.dvol.p, L.flag.p, s->p_singular ? 1 : 0,
+                                    L.dpin, b, x));
         return;
     }
     if (l + 1 == (int)s->mg.size()) {
@@ -371,49 +526,60 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
                                                      L.dxyz.p));
         return;
     }
-    TIMED("mg_zero", cell_b, launch_zero<T>(s->stream, L.d, x));
-    TIMED(fine ? "mg_smooth_fine" : "mg_smooth", 2 * kPre * 3 * cell_b / 2,
-          launch_rbgs<T>(s->stream, L.d, S, G0, flag, b, x, kPre));
-    NormSpec ns{0, nullptr, nullptr, nullptr, nullptr, -1};
-    TIMED(fine ? "mg_resid_fine" : "mg_resid", 3 * cell_b,
-          launch_residual<T>(s->stream, L.d, S, G0, flag, b, x, lev_r<T>(L).p, ns, s->partial.p));
+    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, s->mg_pre, true, fine);
     MGLevel &Cl = s->mg[l + 1];
-    TIMED("mg_restrict", cell_b * 1.125,
-          launch_restrict<T>(s->stream, L.d, Cl.d, Cl.tr, flag, lev_r<T>(L).p, L.dxyz.p + 2,
-                             L.dxyz.p + (L.d.nx + 4) + 2, L.dxyz.p + (L.d.nx + 4) + (L.d.ny + 4) + 2,
-                             Cl.dxyz.p + 2, Cl.dxyz.p + (Cl.d.nx + 4) + 2, Cl.dxyz.p + (Cl.d.nx + 4) + (Cl.d.ny + 4) + 2,
-                             lev_b<T>(Cl).p));
+    // residual restricted in one pass: x and b read, one flag byte per cell
+    TIMED(fine ? "mg_resid_restrict_fine" : "mg_resid_restrict", 2 * cell_b + cell_b / sizeof(T),
+          launch_resid_restrict<T>(s->stream, L.d, S, flag, b, x, Cl.d, Cl.tr, L.dxyz.p, Cl.dxyz.p, lev_b<T>(Cl).p));
     vcycle<T>(s, l + 1, lev_b<T>(Cl).p, lev_x<T>(Cl).p);
     TIMED("mg_prolong", cell_b * 2.125,
           launch_prolong<T>(s->stream, L.d, Cl.d, Cl.tr, flag, Cl.flag.p, lev_x<T>(Cl).p, x));
-    TIMED(fine ? "mg_smooth_fine" : "mg_smooth", 2 * kPost * 3 * cell_b / 2,
-          launch_rbgs<T>(s->stream, L.d, S, G0, flag, b, x, kPost));
+    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, s->mg_post, false, fine);
 }
 
 // V-cycle as a preconditioner. Without per-kernel timing the cycle for a
 // given (input, output) pair is captured once into a HIP graph and replayed:
 // the coarse levels are launch-latency bound, and a graph replay dispatches
 // its kernels back to back without host launch overhead.
-template <typename T>
-static void mg_precond(IBMSolver *s, const T *in, T *out) {
-    if (s->timing) {
-        vcycle<T>(s, 0, in, out);
+template <typename F>
+static void replay_or_capture(IBMSolver *s, const void *in, void *out, F body) {
+    if (s->timing == 1 || !s->graphs) {
+        body();
         return;
     }
     for (auto &g : s->vgraphs)
-        if (g.in == (const void *)in && g.out == (void *)out) {
-            HIP_CHECK(hipGraphLaunch(g.exec, s->stream));
+        if (g.in == in && g.out == out) {
+            TIMED("vcycle_graph", 0, HIP_CHECK(hipGraphLaunch(g.exec, s->stream)));
             return;
         }
     hipGraph_t graph;
+    const int timing = s->timing;
+    s->timing = 0;  // no event records inside the capture
     HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    vcycle<T>(s, 0, in, out);
+    body();
     HIP_CHECK(hipStreamEndCapture(s->stream, &graph));
+    s->timing = timing;
     hipGraphExec_t exec;
     HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     HIP_CHECK(hipGraphDestroy(graph));
-    s->vgraphs.push_back({(const void *)in, (void *)out, exec});
-    HIP_CHECK(hipGraphLaunch(exec, s->stream));
+    s->vgraphs.push_back({in, out, exec});
+    TIMED("vcycle_graph", 0, HIP_CHECK(hipGraphLaunch(exec, s->stream)));
+}
+
+template <typename T>
+static void mg_precond(IBMSolver *s, const T *in, T *out) {
+    replay_or_capture(s, in, out, [&]() { vcycle<T>(s, 0, in, out); });
+}
+
+// FP32 V-cycle applied to an FP64 vector (the preconditioner of the FP64
+// Krylov iteration in mixed precision)
+static void mg_precond_f32(IBMSolver *s, const double *in, double *out) {
+    const double cb = ncells(s);
+    replay_or_capture(s, in, out, [&]() {
+        TIMED("d2f", cb * 12, launch_d2f(s->stream, s->d, in, s->ref_rhs32.p));
+        vcycle<float>(s, 0, s->ref_rhs32.p, s->ref_e32.p);
+        TIMED("f2d", cb * 12, launch_f2d(s->stream, s->d, s->ref_e32.p, out));
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -446,6 +612,9 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
             TIMED("jacobi", 2 * cb, launch_jacobi<T>(s->stream, d, S, G, flag, in, out));
         } else if (prec == 2) {
             mg_precond<T>(s, in, out);
+        } else if (prec == 3) {
+            if constexpr (std::is_same<T, double>::value) mg_precond_f32(s, in, out);
+            else abort();
         } else {
             launch_copy<T>(s->stream, d, in, out);
         }
@@ -528,6 +697,10 @@ static void solve_system(IBMSolver *s, int sk, const double *b, double *x, doubl
     int it;
     if (s->precision == IBM3D_PREC_MIXED) {
         it = refine_mixed(s, sk, b, x, prec, ns, s->tol, maxit, bnorm, &rel);
+    } else if (s->precision == IBM3D_PREC_MIXED_KRYLOV) {
+        const GhostData &GD = sk == 3 ? s->gp : s->gmom;
+        it = bicgstab<double>(s, s->sys[sk].S64, GD.G64, b, x, prec == 2 ? 3 : prec, ns, s->tol, maxit, bnorm, &rel,
+                              sk == 3 ? "p64" : "u64");
     } else {
         const GhostData &GD = sk == 3 ? s->gp : s->gmom;
         it = bicgstab<double>(s, s->sys[sk].S64, GD.G64, b, x, prec, ns, s->tol, maxit, bnorm, &rel,
@@ -729,7 +902,12 @@ using namespace ibm;
 
 extern "C" {
 
-void IBMSolver_enable_timing(IBMSolver *s, int on) { s->timing = on != 0; }
+void IBMSolver_enable_timing(IBMSolver *s, int mode) {
+    if (mode < 0 || mode > 2) fatal("timing mode must be 0, 1 or 2");
+    s->timing = mode;
+}
+
+void IBMSolver_set_graph_replay(IBMSolver *s, int on) { s->graphs = on != 0; }
 
 void IBMSolver_reset_timing(IBMSolver *s) {
     timers_flush(s);

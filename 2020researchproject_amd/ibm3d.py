@@ -33,7 +33,7 @@ FIELDS = ["U1", "U2", "U3", "P", "U1_STAR", "U2_STAR", "U3_STAR", "P_PRIME",
           "N1", "N2", "N3", "LVSET", "FACE_U1", "FACE_U2", "FACE_U3",
           "TAU11", "TAU12", "TAU13", "TAU22", "TAU23", "TAU33"]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_KRYLOV = 0, 1, 2
 
 INIT_FUNC = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_double)
 BC_FUNC = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_double, C.c_double)
@@ -62,6 +62,7 @@ _SIGS = {
     "IBMSolver_ghost_interp": (None, [_vp, C.c_int, C.c_int, C.c_int, _ip, _dp]),
     "IBMSolver_set_obstacle_sphere": (None, [_vp] + [C.c_double] * 4),
     "IBMSolver_set_precision": (None, [_vp, C.c_int]),
+    "IBMSolver_set_mg_params": (None, [_vp, C.c_double, C.c_int, C.c_int, C.c_int]),
     "IBMSolver_get_field": (None, [_vp, C.c_int, _dp]),
     "IBMSolver_set_field": (None, [_vp, C.c_int, _dp]),
     "IBMSolver_get_flag": (None, [_vp, _ip]),
@@ -80,6 +81,7 @@ _SIGS = {
 _OPTIONAL = {
     "IBMSolver_calc_body_force": (None, [_vp, _dp]),
     "IBMSolver_enable_timing": (None, [_vp, C.c_int]),
+    "IBMSolver_set_graph_replay": (None, [_vp, C.c_int]),
     "IBMSolver_get_step_times": (C.c_int, [_vp, _dp, C.c_int]),
     "IBMSolver_reset_timing": (None, [_vp]),
     "IBMSolver_get_kernel_stat": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _dp,
@@ -180,6 +182,9 @@ class Solver:
     def set_precision(self, mode: int):
         self.lib.IBMSolver_set_precision(self.h, mode)
 
+    def set_mg_params(self, gamma: float = 3.0, pre: int = 2, post: int = 2, smoother: int = 1):
+        self.lib.IBMSolver_set_mg_params(self.h, gamma, pre, post, smoother)
+
     def assemble(self):
         self.lib.IBMSolver_assemble(self.h)
 
@@ -255,8 +260,13 @@ class Solver:
         return out
 
     # --- timing (product only) ----------------------------------------------
-    def enable_timing(self, on: bool = True):
-        self.lib.IBMSolver_enable_timing(self.h, int(on))
+    def enable_timing(self, mode: int = 1):
+        """0 off, 1 every kernel (multigrid cycles eager), 2 kernels outside
+        the cycles plus whole graph replays of the cycles."""
+        self.lib.IBMSolver_enable_timing(self.h, int(mode))
+
+    def set_graph_replay(self, on: bool):
+        self.lib.IBMSolver_set_graph_replay(self.h, int(on))
 
     def reset_timing(self):
         self.lib.IBMSolver_reset_timing(self.h)

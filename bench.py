@@ -36,27 +36,27 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(prec: int, nx: int, ny: int, nz: int, steps: int, warmup: int, tol: float, timing: bool):
+def run_gpu(prec: int, nx: int, ny: int, nz: int, steps: int, warmup: int, tol: float, prof_steps: int,
+            graphs: bool = True):
+    """Set up the sphere case, run `warmup` untimed steps, time `steps`
+    steps (no per-kernel instrumentation: multigrid cycles replay as HIP
+    graphs), then run `prof_steps` more steps with every kernel bracketed by
+    HIP events on the solver's stream for the per-kernel table."""
     s = I.Solver()
     t0 = time.perf_counter()
+    s.set_graph_replay(graphs)
     cases.setup_sphere(s, nx, ny, nz, tol=tol, precision=prec)
     log(f"[bench] setup prec={prec} {nx}x{ny}x{nz}: {time.perf_counter() - t0:.2f} s, "
         f"ghost cells {len(s.ghost_cells())}")
     if warmup:
         s.iterate(warmup)
         log(f"[bench] warmup {warmup} steps done (its {s.last_iters().tolist()})")
-    if timing:
-        s.enable_timing(True)
-        s.reset_timing()
     t0 = time.perf_counter()
     s.iterate(steps)
     wall = time.perf_counter() - t0
     step_ms = s.step_times()
-    stats = s.kernel_stats() if timing else []
-    if timing:
-        s.enable_timing(False)
     res = {
-        "wall": wall, "step_ms": step_ms, "stats": stats, "max_div": s.max_div(),
+        "wall": wall, "step_ms": step_ms, "max_div": s.max_div(),
         "iters": s.last_iters().tolist(), "norms": s.last_norms().tolist(),
         "u": [s.get(n)[2:-2, 2:-2, 2:-2] for n in ("U1", "U2", "U3")],
         "p": s.get("P")[2:-2, 2:-2, 2:-2],
@@ -64,6 +64,18 @@ def run_gpu(prec: int, nx: int, ny: int, nz: int, steps: int, warmup: int, tol: 
     }
     log(f"[bench] prec={prec} timed {steps} steps: {wall:.3f} s, max|div U| {res['max_div']:.3e}, "
         f"its {res['iters']}")
+    res["stats"] = []
+    if prof_steps:
+        s.enable_timing(2)
+        s.reset_timing()
+        s.iterate(prof_steps)
+        res["graph_stats"] = s.kernel_stats()
+        s.enable_timing(1)
+        s.reset_timing()
+        s.iterate(prof_steps)
+        res["stats"] = s.kernel_stats()
+        s.enable_timing(0)
+        log(f"[bench] prec={prec} instrumented {prof_steps} steps done")
     s.close()
     return res
 
@@ -73,10 +85,11 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a[m] - b[m]) / max(np.linalg.norm(b[m]), 1e-300))
 
 
-def cpu_baseline(tol: float):
+def cpu_baseline(tol: float, steps: int = 3):
     """Oracle (FP64 CPU restatement, OpenMP, ILU(0)-BiCGSTAB pressure) on a
-    bounded sample of the same workload: one step of the sphere case on the
-    128x64x64 grid (same domain, stretching ratio and physics)."""
+    bounded sample of the same workload: `steps` time steps of the sphere
+    case on the 128x64x64 grid (same domain, stretching law and physics,
+    1/8 of the cells; the full grid takes minutes per step on the CPU)."""
     lib = os.path.join(ROOT, "oracle", "libibm3d_oracle.so")
     if not os.path.exists(lib):
         return None
@@ -84,24 +97,37 @@ def cpu_baseline(tol: float):
     o = I.Solver(lib)
     cases.setup_sphere(o, nx, ny, nz, tol=tol)
     t0 = time.perf_counter()
-    o.iterate(1)
+    o.iterate(steps)
     dt = time.perf_counter() - t0
     o.close()
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": nx * ny * nz / dt / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
-            "sample": f"1 step of the sphere case on {nx}x{ny}x{nz} (1/8 of the cells), "
+    return {"value": nx * ny * nz * steps / dt / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} steps of the sphere case on {nx}x{ny}x{nz} (1/8 of the cells), "
                       f"FP64 oracle, {dt:.2f} s"}
 
 
-def roofline(stats, cells: float):
-    if not stats:
+def roofline(stats):
+    """Roofline entry of the streaming kernel with the largest total time
+    (kernels with a stated algorithmic byte count per launch)."""
+    cand = [t for t in stats if t["bytes"] > 0 and t["count"] > 0]
+    if not cand:
         return None
-    dom = max(stats, key=lambda t: t["ms"])
-    avg_ms = dom["ms"] / max(dom["count"], 1)
-    achieved = dom["bytes"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    return dom, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                 "kernel": dom["name"], "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes"]}
+    dom = max(cand, key=lambda t: t["ms"])
+    avg_ms = dom["ms"] / dom["count"]
+    achieved = dom["bytes"] / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": dom["name"], "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes"],
+            "launches": dom["count"]}
+
+
+def print_stats(tag: str, res) -> None:
+    for key in ("graph_stats", "stats"):
+        for t in sorted(res.get(key, []), key=lambda t: -t["ms"]):
+            avg = t["ms"] / max(t["count"], 1)
+            gbs = t["bytes"] / (avg * 1e-3) / 1e9 if avg > 0 and t["bytes"] else 0
+            log(f"[{tag} {key}] {t['name']:18s} total {t['ms']:9.2f} ms  n {t['count']:6d}  "
+                f"avg {avg * 1e3:9.1f} us  {gbs:8.1f} GB/s")
 
 
 def main() -> int:
@@ -113,7 +139,9 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prof-steps", type=int, default=2, help="instrumented steps for the per-kernel table")
     ap.add_argument("--stats", action="store_true", help="print per-kernel timing to stderr")
+    ap.add_argument("--no-graphs", action="store_true", help="launch multigrid cycles kernel by kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU (one GPU holds the whole domain); --gpus {args.gpus} "
@@ -122,7 +150,8 @@ def main() -> int:
     nx, ny, nz = (int(v) for v in args.grid.split("x"))
     cells = float(nx * ny * nz)
 
-    mixed = run_gpu(I.PREC_MIXED, nx, ny, nz, args.steps, args.warmup, args.tol, timing=True)
+    mixed = run_gpu(I.PREC_MIXED, nx, ny, nz, args.steps, args.warmup, args.tol, args.prof_steps,
+                    not args.no_graphs)
     value = cells * args.steps / mixed["wall"] / 1e6
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
@@ -136,23 +165,23 @@ def main() -> int:
         "max_div": mixed["max_div"],
         "iters_last_step": mixed["iters"],
     }
-    dom_rf = roofline(mixed["stats"], cells)
-    if dom_rf:
-        out["roofline"] = dom_rf[1]
+    rf = roofline(mixed["stats"])
+    if rf:
+        out["roofline"] = rf
     if args.stats:
-        for t in sorted(mixed["stats"], key=lambda t: -t["ms"]):
-            avg = t["ms"] / max(t["count"], 1)
-            gbs = t["bytes"] / (avg * 1e-3) / 1e9 if avg > 0 and t["bytes"] else 0
-            log(f"[stats] {t['name']:18s} total {t['ms']:9.2f} ms  n {t['count']:6d}  avg {avg * 1e3:9.1f} us  "
-                f"{gbs:8.1f} GB/s")
+        print_stats("mixed", mixed)
     if not args.no_fp64:
-        f64 = run_gpu(I.PREC_FP64, nx, ny, nz, args.steps, args.warmup, args.tol, timing=False)
+        f64 = run_gpu(I.PREC_FP64, nx, ny, nz, args.steps, args.warmup, args.tol, args.prof_steps if args.stats else 0,
+                      not args.no_graphs)
         v64 = cells * args.steps / f64["wall"] / 1e6
         out["fp64_value"] = round(v64, 2)
         out["speedup_vs_fp64"] = round(value / v64, 3)
         out["fp64_max_div"] = f64["max_div"]
         out["fp64_p99_ms"] = round(float(np.percentile(f64["step_ms"], 99)), 3)
         out["rel_l2_u_mixed_vs_fp64"] = max(rel_l2(a, b) for a, b in zip(mixed["u"], f64["u"]))
+        out["fp64_iters_last_step"] = f64["iters"]
+        if args.stats:
+            print_stats("fp64", f64)
     if not args.no_cpu:
         cb = cpu_baseline(args.tol)
         if cb:

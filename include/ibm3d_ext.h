@@ -34,6 +34,7 @@ typedef enum _ibm3d_field {
 typedef enum _ibm3d_precision {
     IBM3D_PREC_FP64 = 0,   /* every array and solve in FP64 */
     IBM3D_PREC_MIXED = 1,  /* FP32 smoother / multigrid inside FP64 refinement */
+    IBM3D_PREC_MIXED_KRYLOV = 2, /* FP64 BiCGSTAB, FP32 multigrid preconditioner */
 } IBM3DPrecision;
 
 /* Analytic obstacle phi = |x - c| - r (positive in the fluid). Replaces
@@ -44,6 +45,18 @@ void IBMSolver_set_obstacle_sphere(IBMSolver *, double cx, double cy, double cz,
 /* Precision mode of the linear solves (default IBM3D_PREC_FP64). Must be
    called before IBMSolver_assemble. */
 void IBMSolver_set_precision(IBMSolver *, int mode);
+
+/* Multigrid parameters of the pressure preconditioner (no counterpart in
+   the reference, whose pressure preconditioner is BoomerAMG). gamma: along
+   each axis two neighbouring cells merge on the next level when their
+   combined width is at most 2 * gamma times the smallest cell width of the
+   level (gamma <= 0: merge all pairs of even-sized axes); pre / post:
+   red-black Gauss-Seidel sweeps before / after the coarse correction;
+   smoother 0: each sweep is one fused launch (both colours), ghost rows
+   relaxed after each sweep; 1: one launch per colour, ghost rows relaxed
+   after each colour. Defaults 3.0, 2, 2, 1. Must be called before
+   IBMSolver_assemble. */
+void IBMSolver_set_mg_params(IBMSolver *, double gamma, int pre, int post, int smoother);
 
 /* Copies a field to host memory in the reference layout (see above). Solid
    cells hold NaN where the reference stores NaN. */
@@ -77,8 +90,15 @@ void IBMSolver_calc_body_force(IBMSolver *, double out[3]);
    to max values, returns the number of steps. */
 int IBMSolver_get_step_times(IBMSolver *, double *out, int max);
 
-/* Per-kernel timing with HIP events on the solver's stream. */
-void IBMSolver_enable_timing(IBMSolver *, int on);
+/* Timing with HIP events on the solver's stream. mode 0: off; 1: every
+   kernel launch, multigrid cycles launched eagerly; 2: every launch outside
+   the multigrid cycles plus each replay of a cycle's HIP graph as a whole
+   ("vcycle_graph"). */
+void IBMSolver_enable_timing(IBMSolver *, int mode);
+/* Multigrid cycles are captured once per (input, output) pair into a HIP
+   graph and replayed (default on). Off: every cycle is launched kernel by
+   kernel, e.g. for tracing tools that do not follow graph launches. */
+void IBMSolver_set_graph_replay(IBMSolver *, int on);
 void IBMSolver_reset_timing(IBMSolver *);
 /* Returns the number of timed kernels; fills entry idx when idx is valid:
    name, accumulated milliseconds, launch count, algorithmic bytes moved. */

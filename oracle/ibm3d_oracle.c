@@ -230,6 +230,10 @@ void IBMSolver_set_precision(IBMSolver *s, int mode) {
     (void)s; (void)mode;  /* the oracle is FP64 only */
 }
 
+void IBMSolver_set_mg_params(IBMSolver *s, double gamma, int pre, int post, int smoother) {
+    (void)s; (void)gamma; (void)pre; (void)post; (void)smoother;  /* the oracle has no multigrid */
+}
+
 void IBMSolver_set_linear_solver(IBMSolver *s, IBMSolverLinearSolverType t,
                                  IBMSolverPrecondType pt, const double tol) {
     s->solver_type = t;  /* src/ibm3d_setup.c:461-470 */
