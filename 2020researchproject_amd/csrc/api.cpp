@@ -476,8 +476,18 @@ void build_system(const IBMSolver *s, const std::vector<double> *D[3], const std
             std::vector<float> f(arr[c]->begin(), arr[c]->end());
             out.c32[a][c].upload(f);
         }
-        out.S64.ax[a] = {out.c64[a][0].p, out.c64[a][1].p, out.c64[a][2].p, out.c64[a][3].p, N, periodic};
-        out.S32.ax[a] = {out.c32[a][0].p, out.c32[a][1].p, out.c32[a][2].p, out.c32[a][3].p, N, periodic};
+        {
+            std::vector<double> pk(4 * (size_t)N);
+            for (int i = 0; i < N; i++)
+                for (int c = 0; c < 4; c++) pk[4 * (size_t)i + c] = (*arr[c])[i];
+            out.p64[a].upload(pk);
+            std::vector<float> pf(pk.begin(), pk.end());
+            out.p32[a].upload(pf);
+        }
+        out.S64.ax[a] = {out.c64[a][0].p, out.c64[a][1].p, out.c64[a][2].p, out.c64[a][3].p, N, periodic,
+                         out.p64[a].p};
+        out.S32.ax[a] = {out.c32[a][0].p, out.c32[a][1].p, out.c32[a][2].p, out.c32[a][3].p, N, periodic,
+                         out.p32[a].p};
     }
     out.S64.c0 = sk < 3 ? 1.0 : 0.0;
     out.S32.c0 = sk < 3 ? 1.0f : 0.0f;

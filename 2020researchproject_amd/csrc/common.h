@@ -75,6 +75,18 @@ struct AxisCoef {
     const T *cm, *cp, *dm, *dp;
     int n;
     int periodic;
+    const T *c4;  // [n][4]: cm, cp, dm, dp packed (one vector load per axis)
+};
+
+template <typename T>
+struct Packed4;
+template <>
+struct Packed4<float> {
+    using type = float4;
+};
+template <>
+struct Packed4<double> {
+    using type = double4;
 };
 
 template <typename T>

@@ -12,7 +12,8 @@ namespace ibm {
 // system skips such faces).
 template <typename T>
 __device__ inline void axis_term(const AxisCoef<T> &A, int ii, bool sm, bool sp, T xm, T xp, T &diag, T &off) {
-    T cm = A.cm[ii], cp = A.cp[ii], dm = A.dm[ii], dp = A.dp[ii];
+    const typename Packed4<T>::type c = reinterpret_cast<const typename Packed4<T>::type *>(A.c4)[ii];
+    T cm = c.x, cp = c.y, dm = c.z, dp = c.w;
     if (sm) { cm = 0; dm = 0; }
     if (sp) { cp = 0; dp = 0; }
     diag += dm + dp;

@@ -58,6 +58,8 @@ struct SysData {
     DBuf<double> c64[3][4];   // cm, cp, dm, dp per axis
     DBuf<float> c32[3][4];
     std::vector<double> h[3][4];  // host copies of c64
+    DBuf<double> p64[3];          // packed [n][4] per axis
+    DBuf<float> p32[3];
     SysCoef<double> S64{};
     SysCoef<float> S32{};
     double gamma[3][2] = {};  // RHS fold factor per face
@@ -97,7 +99,7 @@ struct MGLevel {
 
 struct KTimer {
     std::string name;
-    double bytes = 0;       // algorithmic bytes per launch
+    double bytes = 0;       // algorithmic bytes summed over the launches begun
     double total_ms = 0;
     long count = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;

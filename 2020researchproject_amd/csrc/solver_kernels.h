@@ -103,6 +103,10 @@ struct MGTransfer {
     const double *wi, *wj, *wk;   // weight of the parent (1 - weight of the far neighbour)
 };
 
+// r = b - A x on the fluid rows (no norms)
+template <typename T>
+void launch_mg_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
+                        const T *x, T *r);
 // bc = restriction of the fluid rows' residual b - A x (one pass, no r)
 template <typename T>
 void launch_resid_restrict(hipStream_t st, const Dims &df, const SysCoef<T> &S, const uint8_t *flag_f, const T *b,

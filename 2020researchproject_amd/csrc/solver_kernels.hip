@@ -98,7 +98,10 @@ __global__ void __launch_bounds__(256) k_apply(Dims d, SysCoef<T> S, const uint8
     const int k0 = blockIdx.z * kRedZ, k1 = min(k0 + kRedZ, d.nz);
     double acc[2] = {0, 0};
     if (i < d.nx && j < d.ny)
-        for (int k = k0; k < k1; k++) {
+#pragma unroll
+        for (int kk = 0; kk < kRedZ; kk++) {
+            const int k = k0 + kk;
+            if (k >= k1) break;
             const long q = cidx(d, i, j, k);
             const uint8_t f = flag[q];
             bool have = true;
@@ -178,7 +181,10 @@ __global__ void __launch_bounds__(256) k_residual(Dims d, SysCoef<T> S, const ui
     const int k0 = blockIdx.z * kRedZ, k1 = min(k0 + kRedZ, d.nz);
     double acc[2] = {0, 0};
     if (i < d.nx && j < d.ny)
-        for (int k = k0; k < k1; k++) {
+#pragma unroll
+        for (int kk = 0; kk < kRedZ; kk++) {
+            const int k = k0 + kk;
+            if (k >= k1) break;
             const long q = cidx(d, i, j, k);
             const uint8_t f = flag[q];
             bool have = true;
@@ -245,12 +251,7 @@ __global__ void __launch_bounds__(256) k_jacobi(Dims d, SysCoef<T> S, const uint
     const uint8_t f = flag[q];
     if (q == S.pin || f == F_SOLID) { z[q] = r[q]; return; }
     if (f != F_FLUID) return;
-    T diag = S.c0;
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const int ii = a == 0 ? i : a == 1 ? j : k;
-        diag += S.ax[a].dm[ii] + S.ax[a].dp[ii];
-    }
+    const T diag = stencil_diag(S, i, j, k, q);
     z[q] = diag != T(0) ? r[q] / diag : T(0);
 }
 
@@ -302,6 +303,31 @@ __global__ void __launch_bounds__(256) k_rbgs(Dims d, SysCoef<T> S, const uint8_
         stencil(d, S, flag, i, j, k, q, x, diag, off);
     }
     x[q] = diag != T(0) ? (b[q] + off) / diag : T(0);
+}
+
+// r = b - A x on the fluid rows (pinned row: b - x); no norms
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_residual(Dims d, SysCoef<T> S, const uint8_t *flag, const T *b,
+                                                     const T *x, T *r) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    if (flag[q] != F_FLUID) return;
+    T rv;
+    if (q == S.pin) {
+        rv = b[q] - x[q];
+    } else {
+        T diag, off;
+        stencil(d, S, flag, i, j, k, q, x, diag, off);
+        rv = b[q] - (diag * x[q] - off);
+    }
+    r[q] = rv;
+}
+
+template <typename T>
+void launch_mg_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
+                        const T *x, T *r) {
+    hipLaunchKernelGGL((k_mg_residual<T>), igrid(d), dim3(64, 4), 0, st, d, S, flag, b, x, r);
 }
 
 // Residual of the fluid rows restricted to the coarse grid in one pass:
@@ -402,8 +428,8 @@ void launch_rbgs_color(hipStream_t st, const Dims &d, const SysCoef<T> &S, const
 
 template <typename T>
 void launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x) {
+    if (G.ng <= 0) return;
     const int gb = (G.ng + 255) / 256;
-    if (!gb) return;
     hipLaunchKernelGGL((k_ghost_relax<T>), dim3(gb), dim3(256), 0, st, G, is_p, b, x);
     hipLaunchKernelGGL((k_ghost_scatter<T>), dim3(gb), dim3(256), 0, st, G, x);
 }
@@ -455,7 +481,10 @@ __global__ void __launch_bounds__(256) k_bicg_xr(Dims d, const KScal *ks, const 
     double acc[3] = {0, 0, 0};
     if (i < d.nx && j < d.ny) {
         const T alpha = (T)ks->alpha, omega = (T)ks->omega;
-        for (int k = k0; k < k1; k++) {
+#pragma unroll
+        for (int kk = 0; kk < kRedZ; kk++) {
+            const int k = k0 + kk;
+            if (k >= k1) break;
             const long q = cidx(d, i, j, k);
             x[q] = x[q] + alpha * ph[q] + omega * sh[q];
             const T rv = s[q] - omega * t[q];
@@ -644,10 +673,19 @@ __global__ void __launch_bounds__(256) k_restrict(Dims df, Dims dc, MGTransfer t
                                                   const double *dzc, T *bc) {
     const int I = blockIdx.x * 64 + threadIdx.x, J = blockIdx.y * 4 + threadIdx.y, K = blockIdx.z;
     if (I >= dc.nx || J >= dc.ny) return;
+    // one or two children per axis; the fixed 2 x 2 x 2 loop with guards
+    // keeps the child loads independent
+    const int i0 = tr.si[I], ni = tr.si[I + 1] - i0, j0 = tr.sj[J], nj = tr.sj[J + 1] - j0;
+    const int k0 = tr.sk[K], nk = tr.sk[K + 1] - k0;
     double acc = 0;
-    for (int k = tr.sk[K]; k < tr.sk[K + 1]; k++)
-        for (int j = tr.sj[J]; j < tr.sj[J + 1]; j++)
-            for (int i = tr.si[I]; i < tr.si[I + 1]; i++) {
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++)
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                if (a >= ni || bb >= nj || c >= nk) continue;
+                const int i = i0 + a, j = j0 + bb, k = k0 + c;
                 const long q = cidx(df, i, j, k);
                 if (flag_f[q] != F_FLUID) continue;
                 acc += (double)rf[q] * C1(dxf, i) * C1(dyf, j) * C1(dzf, k);
@@ -1096,6 +1134,8 @@ void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, 
                                        const T *, T *, int);                                                      \
     template void launch_rbgs_color<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *, const T *, T *, \
                                        int, int);                                                                 \
+    template void launch_mg_residual<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *, const T *, \
+                                        const T *, T *);                                                          \
     template void launch_resid_restrict<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *,         \
                                            const T *, const T *, const Dims &, const MGTransfer &, const double *, \
                                            const double *, T *);                                                  \

@@ -52,7 +52,7 @@ int timer_begin(IBMSolver *s, const char *name, double bytes) {
         id = (int)s->timers.size() - 1;
         s->timers[id].name = name;
     }
-    s->timers[id].bytes = bytes;
+    s->timers[id].bytes += bytes;
     hipEvent_t a = get_event(s), b = get_event(s);
     HIP_CHECK(hipEventRecord(a, s->stream));
     s->timers[id].pending.push_back({a, b});
@@ -528,9 +528,21 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
     }
     smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, s->mg_pre, true, fine);
     MGLevel &Cl = s->mg[l + 1];
-    // residual restricted in one pass: x and b read, one flag byte per cell
-    TIMED(fine ? "mg_resid_restrict_fine" : "mg_resid_restrict", 2 * cell_b + cell_b / sizeof(T),
-          launch_resid_restrict<T>(s->stream, L.d, S, flag, b, x, Cl.d, Cl.tr, L.dxyz.p, Cl.dxyz.p, lev_b<T>(Cl).p));
+    if (fine) {
+        // fine level: residual, then restriction (measured faster than the
+        // one-pass kernel, whose per-coarse-cell child loops run short here)
+        TIMED("mg_resid_fine", 3 * cell_b + cell_b / sizeof(T),
+              launch_mg_residual<T>(s->stream, L.d, S, flag, b, x, lev_r<T>(L).p));
+        TIMED("mg_restrict", cell_b + cell_b / sizeof(T),
+              launch_restrict<T>(s->stream, L.d, Cl.d, Cl.tr, flag, lev_r<T>(L).p, L.dxyz.p, L.dxyz.p + (L.d.nx + 4),
+                                 L.dxyz.p + (L.d.nx + 4) + (L.d.ny + 4), Cl.dxyz.p, Cl.dxyz.p + (Cl.d.nx + 4),
+                                 Cl.dxyz.p + (Cl.d.nx + 4) + (Cl.d.ny + 4), lev_b<T>(Cl).p));
+    } else {
+        // residual restricted in one pass: x and b read, one flag byte per cell
+        TIMED("mg_resid_restrict", 2 * cell_b + cell_b / sizeof(T),
+              launch_resid_restrict<T>(s->stream, L.d, S, flag, b, x, Cl.d, Cl.tr, L.dxyz.p, Cl.dxyz.p,
+                                       lev_b<T>(Cl).p));
+    }
     vcycle<T>(s, l + 1, lev_b<T>(Cl).p, lev_x<T>(Cl).p);
     TIMED("mg_prolong", cell_b * 2.125,
           launch_prolong<T>(s->stream, L.d, Cl.d, Cl.tr, flag, Cl.flag.p, lev_x<T>(Cl).p, x));
@@ -922,7 +934,7 @@ int IBMSolver_get_kernel_stat(IBMSolver *s, int idx, char *name, int namelen, do
         if (name && namelen > 0) { strncpy(name, t.name.c_str(), namelen - 1); name[namelen - 1] = 0; }
         if (total_ms) *total_ms = t.total_ms;
         if (count) *count = t.count;
-        if (bytes) *bytes = t.bytes;
+        if (bytes) *bytes = t.count > 0 ? t.bytes / t.count : 0.0;  // average per launch
     }
     return (int)s->timers.size();
 }

@@ -101,7 +101,7 @@ void IBMSolver_enable_timing(IBMSolver *, int mode);
 void IBMSolver_set_graph_replay(IBMSolver *, int on);
 void IBMSolver_reset_timing(IBMSolver *);
 /* Returns the number of timed kernels; fills entry idx when idx is valid:
-   name, accumulated milliseconds, launch count, algorithmic bytes moved. */
+   name, accumulated milliseconds, launch count, algorithmic bytes per launch (averaged over the launches). */
 int IBMSolver_get_kernel_stat(IBMSolver *, int idx, char *name, int namelen,
                               double *total_ms, long *count, double *bytes);
 

@@ -99,7 +99,7 @@ template <typename T>
 static bool one_case(std::mt19937_64 &rng, int nx, int ny, int nz, int per, int skip, int pin, int zero_in) {
     const Dims d = make_dims(nx, ny, nz);
     std::uniform_real_distribution<double> U(0.1, 1.0), V(-1.0, 1.0);
-    std::vector<std::vector<T>> coef(12);
+    std::vector<std::vector<T>> coef(12), packed(3);
     const int n[3] = {nx, ny, nz};
     SysCoef<T> S{};
     for (int a = 0; a < 3; a++) {
@@ -107,8 +107,11 @@ static bool one_case(std::mt19937_64 &rng, int nx, int ny, int nz, int per, int 
             coef[4 * a + c].resize(n[a]);
             for (auto &v : coef[4 * a + c]) v = (T)U(rng);
         }
+        packed[a].resize(4 * (size_t)n[a]);
+        for (int i = 0; i < n[a]; i++)
+            for (int c = 0; c < 4; c++) packed[a][4 * (size_t)i + c] = coef[4 * a + c][i];
         S.ax[a] = {coef[4 * a].data(), coef[4 * a + 1].data(), coef[4 * a + 2].data(), coef[4 * a + 3].data(), n[a],
-                   (per >> a) & 1};
+                   (per >> a) & 1, packed[a].data()};
     }
     S.c0 = (T)0.25;
     S.skip = skip;
