@@ -114,9 +114,13 @@ def test_stl_sphere_flags_bit_exact(new_gpu, new_oracle, I, cases):
         compare(g, o, 1e-8, names=("U1", "U2", "U3", "P"))
 
 
-def test_cavity_mixed_precision(new_gpu, new_oracle, cases, I):
+MIXED_MODES = pytest.mark.parametrize("mode", [1, 2], ids=["defect-correction", "fp64-krylov-fp32-mg"])
+
+
+@MIXED_MODES
+def test_cavity_mixed_precision(new_gpu, new_oracle, cases, I, mode):
     g, o = new_gpu(), new_oracle()
-    cases.setup_cavity(g, n=24, tol=1e-10, precision=I.PREC_MIXED)
+    cases.setup_cavity(g, n=24, tol=1e-10, precision=mode)
     cases.setup_cavity(o, n=24, tol=1e-12)
     g.iterate(5)
     o.iterate(5)
@@ -125,9 +129,10 @@ def test_cavity_mixed_precision(new_gpu, new_oracle, cases, I):
     assert (g.last_norms() <= 1e-10).all()
 
 
-def test_sphere_mixed_precision(new_gpu, new_oracle, cases, I):
+@MIXED_MODES
+def test_sphere_mixed_precision(new_gpu, new_oracle, cases, I, mode):
     g, o = new_gpu(), new_oracle()
-    cases.setup_sphere(g, 48, 24, 24, tol=1e-10, precision=I.PREC_MIXED)
+    cases.setup_sphere(g, 48, 24, 24, tol=1e-10, precision=mode)
     cases.setup_sphere(o, 48, 24, 24, tol=1e-12)
     g.iterate(4)
     o.iterate(4)
@@ -276,3 +281,28 @@ def test_sphere_runs_are_bitwise_reproducible(new_gpu, cases, precision):
     assert runs[0][0] == runs[1][0]
     assert np.array_equal(runs[0][1], runs[1][1], equal_nan=True)
     assert np.array_equal(runs[0][2], runs[1][2], equal_nan=True)
+
+
+@pytest.mark.parametrize("smoother", [0, 1], ids=["fused-sweep", "per-colour"])
+def test_multigrid_smoothers_reach_the_same_solution(new_gpu, cases, smoother):
+    """Both red-black smoothers (one fused launch per sweep, or one launch
+    per colour) are preconditioners only: the converged fields agree with
+    the per-colour default to well below the solver tolerance."""
+    ref, g = new_gpu(), new_gpu()
+    cases.setup_sphere(ref, 64, 32, 32, tol=1e-12)
+    g.set_mg_params(3.0, 2, 2, smoother)
+    cases.setup_sphere(g, 64, 32, 32, tol=1e-12)
+    ref.iterate(3)
+    g.iterate(3)
+    compare(g, ref, 1e-9, names=("U1", "U2", "U3", "P"))
+
+
+@pytest.mark.parametrize("precision", [0, 2], ids=["fp64", "fp64-krylov-fp32-mg"])
+def test_divergence_bound_at_bench_tolerance(new_gpu, cases, precision):
+    """At the benchmark's solver tolerance (1e-9) the projected face
+    velocities satisfy max|div U| <= 1e-8, the common bar of both paths."""
+    g = new_gpu()
+    cases.setup_sphere(g, 128, 64, 64, tol=1e-9, precision=precision)
+    g.iterate(3)
+    assert g.max_div() <= 1e-8
+    assert (g.last_norms() <= 1e-9).all()
