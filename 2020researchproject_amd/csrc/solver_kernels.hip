@@ -374,7 +374,7 @@ void launch_resid_restrict(hipStream_t st, const Dims &df, const SysCoef<T> &S, 
 static int rb_kchunk(const Dims &d) {
     const long tiles = (long)((d.nx + kRbTX - 1) / kRbTX) * ((d.ny + kRbTY - 1) / kRbTY);
     long kc = (tiles * d.nz + 1023) / 1024;
-    kc = std::max(4L, std::min((long)d.nz, kc));
+    kc = std::max(4L, std::min(std::min((long)d.nz, (long)kRbMaxChunk), kc));
     return (int)kc;
 }
 
