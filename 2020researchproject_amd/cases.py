@@ -136,9 +136,11 @@ CAVITY_SEED = 2020
 
 def setup_cavity(s: I.Solver, n: int = 32, Re: float = 100.0, dt: float = 0.005,
                  tol: float = 1e-10, seed: int = CAVITY_SEED, amp: float = 1e-3,
-                 precision: int = I.PREC_FP64, init: str = "array"):
-    f = uniform_faces(0.0, 1.0, n)
-    s.set_grid(f, f, f)
+                 precision: int = I.PREC_FP64, init: str = "array", shape=None):
+    """Lid-driven cavity on the unit cube, n^3 cells (or shape = (nx, ny, nz))."""
+    nx, ny, nz = shape if shape is not None else (n, n, n)
+    fx, fy, fz = uniform_faces(0.0, 1.0, nx), uniform_faces(0.0, 1.0, ny), uniform_faces(0.0, 1.0, nz)
+    s.set_grid(fx, fy, fz)
     s.set_params(Re, dt)
     s.set_bc(I.DIR_NORTH, I.BC_VELOCITY_COMPONENT, I.VAL_CONST, 1.0, 0.0, 0.0)
     s.set_bc(I.DIR_EAST | I.DIR_WEST | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP, I.BC_STATIONARY_WALL)
@@ -158,7 +160,7 @@ def setup_cavity(s: I.Solver, n: int = 32, Re: float = 100.0, dt: float = 0.005,
         s.init_flow_func(pert(1), pert(2), pert(3), lambda x, y, z: 0.0)
     else:
         s.init_flow_const(0.0, 0.0, 0.0, 0.0)
-        xc, yc, zc = cell_centres(f), cell_centres(f), cell_centres(f)
+        xc, yc, zc = cell_centres(fx), cell_centres(fy), cell_centres(fz)
         inside = (((xc > 0) & (xc < 1))[:, None, None] & ((yc > 0) & (yc < 1))[None, :, None]
                   & ((zc > 0) & (zc < 1))[None, None, :])
         for comp, name in ((1, "U1"), (2, "U2"), (3, "U3")):

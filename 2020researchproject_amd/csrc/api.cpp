@@ -48,7 +48,9 @@ IBMSolver *IBMSolver_new(const int num_process, const int rank, const int Px, co
                 num_process, Px, Py, Pz);
         abort();
     }
+    HipGlobalLock lk(hip_global_mutex());
     IBMSolver *s = new IBMSolver();
+    DevBytesScope account(&s->dev_bytes);
     HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_CHECK(hipHostMalloc((void **)&s->ks_h, sizeof(KScal), hipHostMallocDefault));
     HIP_CHECK(hipHostMalloc((void **)&s->sums_h, 16 * sizeof(double), hipHostMallocDefault));
@@ -60,6 +62,7 @@ IBMSolver *IBMSolver_new(const int num_process, const int rank, const int Px, co
 void IBMSolver_destroy(IBMSolver *s) {
     if (!s) return;
     (void)hipStreamSynchronize(s->stream);
+    HipGlobalLock lk(hip_global_mutex());
     for (auto &t : s->timers)
         for (auto &e : t.pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (auto e : s->event_pool) (void)hipEventDestroy(e);
@@ -540,6 +543,7 @@ static void alloc_fields(IBMSolver *s) {
 
 extern "C" void IBMSolver_assemble(IBMSolver *s) {
     if (s->Nx == 0) fatal("IBMSolver_set_grid must precede IBMSolver_assemble");
+    DevBytesScope account(&s->dev_bytes);
     // <- src/ibm3d_setup.c:523-533
     if (isperiodic(s->bc[0].type) != isperiodic(s->bc[2].type) || isperiodic(s->bc[1].type) != isperiodic(s->bc[3].type)
         || isperiodic(s->bc[4].type) != isperiodic(s->bc[5].type)) {
@@ -599,7 +603,7 @@ extern "C" void IBMSolver_assemble(IBMSolver *s) {
     alloc_fields(s);
     build_mg(s);
     s->assembled = true;
-    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipStreamSynchronize(s->stream));
 }
 
 // ---------------------------------------------------------------------------
@@ -676,7 +680,8 @@ extern "C" void IBMSolver_get_field(IBMSolver *s, int field, double *out) {
     std::vector<double> h(d.ncell);
     if (field >= IBM3D_FIELD_FACE_U1 && field <= IBM3D_FIELD_FACE_U3) {
         const int a = field - IBM3D_FIELD_FACE_U1;
-        HIP_CHECK(hipMemcpy(h.data(), s->F[a].p, d.ncell * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpyAsync(h.data(), s->F[a].p, d.ncell * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
         if (a == 0) {
             for (int i = 0; i <= Nx; i++)
                 for (int j = -2; j < Ny + 2; j++)
@@ -702,7 +707,8 @@ extern "C" void IBMSolver_get_field(IBMSolver *s, int field, double *out) {
         for (long q = 0; q < P; q++) out[q] = 0;
         return;
     }
-    HIP_CHECK(hipMemcpy(h.data(), src->p, d.ncell * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpyAsync(h.data(), src->p, d.ncell * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
     for (int i = -2; i < Nx + 2; i++)
         for (int j = -2; j < Ny + 2; j++)
             for (int k = -2; k < Nz + 2; k++) out[host_ref_index(s, i, j, k)] = h[cidx(d, i, j, k)];

@@ -393,6 +393,8 @@ void polyhedron_cpt_grid(const Polyhedron *poly, int nx, int ny, int nz, const d
                          const double *z, double *f, double maxd) {
     const std::vector<CptFeature> feat = build_features(poly, maxd);
     const long n = (long)nx * ny * nz;
+    // legacy-stream launches and copies: keep other threads' graph captures out
+    HipGlobalLock lk(hip_global_mutex());
     DBuf<double> dx, dy, dz, df;
     DBuf<CptFeature> dfeat;
     dx.upload(x, nx); dy.upload(y, ny); dz.upload(z, nz);

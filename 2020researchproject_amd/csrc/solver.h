@@ -2,6 +2,7 @@
 #pragma once
 
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/ibm3d.h"
@@ -12,6 +13,27 @@
 namespace ibm {
 
 // Device buffer owned by the solver.
+// Process-wide lock around HIP calls that touch the legacy default stream or
+// the allocator (hipMemset / hipMemcpy / hipMalloc / hipFree, stream and
+// event creation) and around stream capture: HIP rejects legacy-stream work
+// issued by one thread while another thread captures a graph, and several
+// solvers may run on different threads (the serving loop).
+inline std::recursive_mutex &hip_global_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+using HipGlobalLock = std::lock_guard<std::recursive_mutex>;
+
+// Device bytes allocated by the solver whose API call is running on this
+// thread (set by a scope guard in the setup entry points); null elsewhere.
+inline thread_local size_t *tl_dev_bytes = nullptr;
+
+struct DevBytesScope {
+    size_t *prev;
+    explicit DevBytesScope(size_t *c) : prev(tl_dev_bytes) { tl_dev_bytes = c; }
+    ~DevBytesScope() { tl_dev_bytes = prev; }
+};
+
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -26,20 +48,25 @@ struct DBuf {
     }
     ~DBuf() { free(); }
     void alloc(size_t cnt) {
+        HipGlobalLock lk(hip_global_mutex());
         free();
         n = cnt;
         if (cnt) {
             HIP_CHECK(hipMalloc(&p, cnt * sizeof(T)));
             HIP_CHECK(hipMemset(p, 0, cnt * sizeof(T)));
+            if (tl_dev_bytes) *tl_dev_bytes += cnt * sizeof(T);
         }
     }
     void upload(const T *h, size_t cnt) {
+        HipGlobalLock lk(hip_global_mutex());
         if (cnt > n) alloc(cnt);
         if (cnt) HIP_CHECK(hipMemcpy(p, h, cnt * sizeof(T), hipMemcpyHostToDevice));
     }
     void upload(const std::vector<T> &h) { upload(h.data(), h.size()); }
     void free() {
-        if (p) (void)hipFree(p);
+        if (!p) return;
+        HipGlobalLock lk(hip_global_mutex());
+        (void)hipFree(p);
         p = nullptr;
         n = 0;
     }
@@ -145,6 +172,7 @@ struct _ibm_solver {
     int precision = IBM3D_PREC_FP64;
     bool assembled = false;
     bool p_singular = true;
+    size_t dev_bytes = 0;        // device memory allocated during setup
     double mg_gamma = 3.0;       // coarsening threshold (multiples of the level's smallest width)
     int mg_pre = 2, mg_post = 2; // smoothing sweeps per level
     int mg_smoother = 1;         // 0 fused red-black sweeps, 1 one launch per colour

@@ -38,6 +38,7 @@ static hipEvent_t get_event(IBMSolver *s) {
         return e;
     }
     hipEvent_t e;
+    HipGlobalLock lk(hip_global_mutex());
     HIP_CHECK(hipEventCreate(&e));
     return e;
 }
@@ -565,15 +566,18 @@ static void replay_or_capture(IBMSolver *s, const void *in, void *out, F body) {
             return;
         }
     hipGraph_t graph;
-    const int timing = s->timing;
-    s->timing = 0;  // no event records inside the capture
-    HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    body();
-    HIP_CHECK(hipStreamEndCapture(s->stream, &graph));
-    s->timing = timing;
     hipGraphExec_t exec;
-    HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    HIP_CHECK(hipGraphDestroy(graph));
+    {
+        HipGlobalLock lk(hip_global_mutex());
+        const int timing = s->timing;
+        s->timing = 0;  // no event records inside the capture
+        HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        body();
+        HIP_CHECK(hipStreamEndCapture(s->stream, &graph));
+        s->timing = timing;
+        HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        HIP_CHECK(hipGraphDestroy(graph));
+    }
     s->vgraphs.push_back({in, out, exec});
     TIMED("vcycle_graph", 0, HIP_CHECK(hipGraphLaunch(exec, s->stream)));
 }

@@ -86,6 +86,14 @@ _OPTIONAL = {
     "IBMSolver_reset_timing": (None, [_vp]),
     "IBMSolver_get_kernel_stat": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _dp,
                                             C.POINTER(C.c_long), _dp]),
+    "IBMSolver_get_device_bytes": (C.c_double, [_vp]),
+    "IBMSolver_estimate_device_bytes": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "IBMServer_new": (_vp, [C.c_double, C.c_int]),
+    "IBMServer_submit": (C.c_int, [_vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, _vp]),
+    "IBMServer_wait": (None, [_vp]),
+    "IBMServer_job_times": (C.c_int, [_vp, C.c_int, _dp]),
+    "IBMServer_peak_bytes": (C.c_double, [_vp]),
+    "IBMServer_destroy": (None, [_vp]),
 }
 
 _LIBS: dict[str, C.CDLL] = {}
@@ -289,6 +297,15 @@ class Solver:
             out.append({"name": name.value.decode(), "ms": ms.value, "count": cnt.value,
                         "bytes": nbytes.value})
         return out
+
+    def device_bytes(self) -> float:
+        return self.lib.IBMSolver_get_device_bytes(self.h)
+
+    def detach(self):
+        """Hands the handle over to a caller that will destroy it (e.g. the
+        server); this object no longer owns it."""
+        h, self.h = self.h, None
+        return h
 
     def close(self):
         if self.h:

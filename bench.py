@@ -151,6 +151,36 @@ def copy_calibration(mib: float) -> None:
     del a, b
 
 
+def run_serving(args) -> int:
+    """Serving stream: n independent cases (lid-driven cavity or analytic
+    sphere with probability 1/2; grids 32^3, 48^3, 64^3, 96^3, 128x64x64; Re
+    ~ U[100, 1000]; 100-400 steps; seed 5) arriving with exponential gaps of
+    mean 50 ms, mixed precision, pressure tolerance 1e-10, through the native
+    request loop. Prints one JSON line: aggregate Mcell-steps/s and the
+    p50/p99 time-to-solution (completion - arrival)."""
+    serving = importlib.import_module("2020researchproject_amd.serving")
+    specs = serving.make_stream(n=args.serving_cases, seed=5)
+    budget = args.serving_budget_gib * (1 << 30)
+    t0 = time.perf_counter()
+    res = serving.run_stream(specs, budget_bytes=budget, max_active=args.serving_slots)
+    wall = time.perf_counter() - t0
+    work = sum(c.cells * c.steps for c in specs)
+    out = {
+        "metric": "serving stream: aggregate Mcell-steps/s, p50/p99 time-to-solution",
+        "value": round(res["mcell_steps_per_s"], 2), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": int(sum(c.steps for c in specs)), "warmup": 0, "higher_is_better": True, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{len(specs)} cases, Poisson arrivals (mean 50 ms), cavity/sphere, "
+                               "grids 32^3..128x64x64, Re 100-1000, 100-400 steps, tol 1e-10",
+                   "slots": args.serving_slots, "budget_gib": args.serving_budget_gib},
+        "p50_s": round(res["p50_s"], 3), "p99_s": round(res["p99_s"], 3),
+        "makespan_s": round(res["makespan_s"], 3), "wall_s": round(wall, 3),
+        "cell_steps": work, "peak_admitted_gib": round(res["peak_admitted_bytes"] / (1 << 30), 3),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def print_stats(tag: str, res) -> None:
     for key in ("graph_stats", "stats"):
         for t in sorted(res.get(key, []), key=lambda t: -t["ms"]):
@@ -176,11 +206,18 @@ def main() -> int:
     ap.add_argument("--prof-steps", type=int, default=2, help="instrumented steps for the per-kernel table")
     ap.add_argument("--stats", action="store_true", help="print per-kernel timing to stderr")
     ap.add_argument("--no-graphs", action="store_true", help="launch multigrid cycles kernel by kernel")
+    ap.add_argument("--serving", action="store_true",
+                    help="run the serving stream (64 cavity/sphere cases, Poisson arrivals) instead")
+    ap.add_argument("--serving-cases", type=int, default=64)
+    ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-budget-gib", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU (one GPU holds the whole domain); --gpus {args.gpus} "
               "is not supported", file=sys.stderr)
         return 2
+    if args.serving:
+        return run_serving(args)
     nx, ny, nz = (int(v) for v in args.grid.split("x"))
     cells = float(nx * ny * nz)
 
