@@ -74,6 +74,7 @@ _SIGS = {
     "IBMSolver_max_div": (C.c_double, [_vp]),
     "Polyhedron_new": (_vp, []),
     "Polyhedron_destroy": (None, [_vp]),
+    "Polyhedron_read_stl": (None, [_vp, _vp]),
 }
 # variadic: IBMSolver_set_bc, IBMSolver_set_ext_force, IBMSolver_set_turb_model
 
@@ -184,6 +185,24 @@ class Solver:
     def set_obstacle(self, poly):
         self.lib.IBMSolver_set_obstacle(self.h, poly)
 
+    def set_obstacle_stl(self, path: str):
+        """Reads an STL file with Polyhedron_read_stl and sets it as the
+        obstacle; the polyhedron is destroyed after assemble (it is borrowed
+        until then, as in the reference)."""
+        import ctypes.util
+        libc = C.CDLL(ctypes.util.find_library("c"))
+        libc.fopen.restype = C.c_void_p
+        libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+        libc.fclose.argtypes = [C.c_void_p]
+        fp = libc.fopen(os.fsencode(path), b"rb")
+        if not fp:
+            raise FileNotFoundError(path)
+        poly = self.lib.Polyhedron_new()
+        self.lib.Polyhedron_read_stl(poly, fp)
+        libc.fclose(fp)
+        self._poly = poly
+        self.lib.IBMSolver_set_obstacle(self.h, poly)
+
     def set_linear_solver(self, solver=SOLVER_BiCGSTAB, precond=PRECOND_AMG, tol=1e-10):
         self.lib.IBMSolver_set_linear_solver(self.h, solver, precond, tol)
 
@@ -195,6 +214,10 @@ class Solver:
 
     def assemble(self):
         self.lib.IBMSolver_assemble(self.h)
+        poly = getattr(self, "_poly", None)
+        if poly:
+            self.lib.Polyhedron_destroy(poly)
+            self._poly = None
 
     # --- init / run --------------------------------------------------------
     def init_flow_const(self, u1=0.0, u2=0.0, u3=0.0, p=0.0):
