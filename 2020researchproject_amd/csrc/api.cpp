@@ -118,6 +118,12 @@ void IBMSolver_set_turb_model(IBMSolver *s, IBMSolverTurbModelType t, ...) {
     case TURBMODEL_NONE: break;
     case TURBMODEL_SMAGORINSKY:
     case TURBMODEL_VREMAN: s->turb_coef = va_arg(ap, double); break;
+    case TURBMODEL_SMAGORINSKY_VD:
+        s->turb_coef = va_arg(ap, double);
+        s->vd_aplus = va_arg(ap, double);
+        s->vd_utau = va_arg(ap, double);
+        s->vd_walls = va_arg(ap, int);
+        break;
     default:
         va_end(ap);
         fatal("unsupported turbulence model");

@@ -173,6 +173,8 @@ struct _ibm_solver {
     bool assembled = false;
     bool p_singular = true;
     size_t dev_bytes = 0;        // device memory allocated during setup
+    double vd_aplus = 26, vd_utau = 1;  // van Driest damping (TURBMODEL_SMAGORINSKY_VD)
+    int vd_walls = 0;
     double mg_gamma = 3.0;       // coarsening threshold (multiples of the level's smallest width)
     int mg_pre = 2, mg_post = 2; // smoothing sweeps per level
     int mg_smoother = 1;         // 0 fused red-black sweeps, 1 one launch per colour

@@ -771,9 +771,17 @@ static TauPtrs tau_ptrs(IBMSolver *s) {
 static void calc_tau(IBMSolver *s) {
     if (s->turb_type == TURBMODEL_NONE) return;
     const double *u3[3] = {s->u[0].p, s->u[1].p, s->u[2].p};
+    VanDriest vd{};
+    const bool smag = s->turb_type == TURBMODEL_SMAGORINSKY || s->turb_type == TURBMODEL_SMAGORINSKY_VD;
+    if (s->turb_type == TURBMODEL_SMAGORINSKY_VD) {
+        vd.walls = s->vd_walls;
+        vd.aplus = s->vd_aplus;
+        vd.ure = s->vd_utau * s->Re;
+        vd.lo[0] = s->xmin; vd.lo[1] = s->ymin; vd.lo[2] = s->zmin;
+        vd.hi[0] = s->xmax; vd.hi[1] = s->ymax; vd.hi[2] = s->zmax;
+    }
     TIMED("sgs", ncells(s) * (3 + 6) * 8 + ncells(s),
-          launch_sgs(s->stream, s->geo, s->turb_type == TURBMODEL_SMAGORINSKY ? 1 : 2, s->turb_coef, s->flag.p,
-                     s->lvset.p, u3, tau_ptrs(s)));
+          launch_sgs(s->stream, s->geo, smag ? 1 : 2, s->turb_coef, vd, s->flag.p, s->lvset.p, u3, tau_ptrs(s)));
 }
 
 static void do_step(IBMSolver *s) {

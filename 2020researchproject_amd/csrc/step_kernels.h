@@ -18,6 +18,15 @@ struct StepParams {
     const double *fa[3];   // per-cell body force (host-evaluated), or null
 };
 
+// van Driest damping of the Smagorinsky length: walls = direction bits
+// (N 1, E 2, S 4, W 8, D 16, U 32) of the outer faces that count as walls;
+// lo / hi: domain bounds; ure = u_tau * Re (1 / nu = Re).
+struct VanDriest {
+    int walls;
+    double aplus, ure;
+    double lo[3], hi[3];
+};
+
 struct TauPtrs {
     int on;
     double *t[6];          // 11, 12, 13, 22, 23, 33
@@ -65,8 +74,8 @@ void launch_ghost_update(hipStream_t st, const GhostCell *gc, int ng, const int 
                          double *old, double *u1, double *u2, double *u3, double *p);
 void launch_solid_nan(hipStream_t st, const Dims &d, const uint8_t *flag, double *u1, double *u2, double *u3,
                       double *p);
-void launch_sgs(hipStream_t st, const Geo &g, int model, double coef, const uint8_t *flag, const double *lvset,
-                const double *const u[3], const TauPtrs &tau);
+void launch_sgs(hipStream_t st, const Geo &g, int model, double coef, const VanDriest &vd, const uint8_t *flag,
+                const double *lvset, const double *const u[3], const TauPtrs &tau);
 long maxdiv_blocks(const Dims &d);
 void launch_maxdiv(hipStream_t st, const Geo &g, const uint8_t *flag, const double *const U[3], double *partial);
 

@@ -469,7 +469,7 @@ __global__ void k_solid_nan(Dims d, const uint8_t *flag, double *u1, double *u2,
 // 0.41 phi), Delta = 2 cbrt(dx dy dz)); Vreman (extension) with
 // nu_t = c sqrt(B / (alpha:alpha)). tau = -2 nu_t S on fluid cells only.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_sgs(Geo g, int model, double coef, const uint8_t *flag,
+__global__ void __launch_bounds__(256) k_sgs(Geo g, int model, double coef, VanDriest vd, const uint8_t *flag,
                                              const double *lvset, const double *u1, const double *u2,
                                              const double *u3, TauPtrs tau) {
     const Dims &d = g.d;
@@ -507,6 +507,18 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, int model, double coef, cons
             + 2 * S[3] * S[3] + 2 * S[4] * S[4] + 2 * S[2] * S[2] + 2 * S[4] * S[4] + 2 * S[5] * S[5];
         const double smag = sqrt(s2);
         double l = coef * delta;
+        if (vd.walls) {
+            // van Driest: l *= 1 - exp(-y+ / A+), y+ = y_w u_tau / nu
+            const double c[3] = {C1(g.xc, i), C1(g.yc, j), C1(g.zc, k)};
+            const int bit_lo[3] = {8, 4, 16}, bit_hi[3] = {2, 1, 32};  // W S D, E N U
+            double yw = 1e300;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                if (vd.walls & bit_lo[a]) yw = fmin(yw, c[a] - vd.lo[a]);
+                if (vd.walls & bit_hi[a]) yw = fmin(yw, vd.hi[a] - c[a]);
+            }
+            l *= 1 - exp(-(yw * vd.ure) / vd.aplus);
+        }
         const double lw = 0.41 * lvset[q];
         if (lw < l) l = lw;
         const double f = -2 * (l * l) * smag;
@@ -648,10 +660,10 @@ void launch_solid_nan(hipStream_t st, const Dims &d, const uint8_t *flag, double
     hipLaunchKernelGGL(k_solid_nan, cell_grid(d, 0), dim3(64, 4), 0, st, d, flag, u1, u2, u3, p);
 }
 
-void launch_sgs(hipStream_t st, const Geo &g, int model, double coef, const uint8_t *flag, const double *lvset,
-                const double *const u[3], const TauPtrs &tau) {
-    hipLaunchKernelGGL(k_sgs, cell_grid(g.d, 0), dim3(64, 4), 0, st, g, model, coef, flag, lvset, u[0], u[1], u[2],
-                       tau);
+void launch_sgs(hipStream_t st, const Geo &g, int model, double coef, const VanDriest &vd, const uint8_t *flag,
+                const double *lvset, const double *const u[3], const TauPtrs &tau) {
+    hipLaunchKernelGGL(k_sgs, cell_grid(g.d, 0), dim3(64, 4), 0, st, g, model, coef, vd, flag, lvset, u[0], u[1],
+                       u[2], tau);
 }
 
 long maxdiv_blocks(const Dims &d) { return conv_rhs_blocks(d); }

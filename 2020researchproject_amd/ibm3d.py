@@ -25,7 +25,7 @@ DIR_NORTH, DIR_EAST, DIR_SOUTH, DIR_WEST, DIR_DOWN, DIR_UP = (1 << i for i in ra
 (BC_VELOCITY_COMPONENT, BC_PRESSURE, BC_STATIONARY_WALL, BC_FREE_SLIP_WALL,
  BC_SYMMETRIC, BC_ALL_PERIODIC, BC_VELOCITY_PERIODIC) = range(7)
 VAL_CONST, VAL_FUNC = range(2)
-TURBMODEL_NONE, TURBMODEL_SMAGORINSKY, TURBMODEL_WALE, TURBMODEL_VREMAN = range(4)
+TURBMODEL_NONE, TURBMODEL_SMAGORINSKY, TURBMODEL_WALE, TURBMODEL_VREMAN, TURBMODEL_SMAGORINSKY_VD = range(5)
 FLAG_FLUID, FLAG_GHOST, FLAG_SOLID = 1, 2, 3
 
 # include/ibm3d_ext.h
@@ -173,9 +173,15 @@ class Solver:
             self._keep += cbs
             self.lib.IBMSolver_set_ext_force(C.c_void_p(self.h), C.c_int(VAL_FUNC), *[C.cast(c, _vp) for c in cbs])
 
-    def set_turb_model(self, model: int, coef: float = 0.0):
+    def set_turb_model(self, model: int, coef: float = 0.0, aplus: float = 26.0, u_tau: float = 1.0,
+                       walls: int = 0):
+        """coef: Cs (Smagorinsky) or c (Vreman); aplus, u_tau, walls: van
+        Driest damping of TURBMODEL_SMAGORINSKY_VD."""
         if model == TURBMODEL_NONE:
             self.lib.IBMSolver_set_turb_model(C.c_void_p(self.h), C.c_int(model))
+        elif model == TURBMODEL_SMAGORINSKY_VD:
+            self.lib.IBMSolver_set_turb_model(C.c_void_p(self.h), C.c_int(model), C.c_double(coef),
+                                              C.c_double(aplus), C.c_double(u_tau), C.c_int(walls))
         else:
             self.lib.IBMSolver_set_turb_model(C.c_void_p(self.h), C.c_int(model), C.c_double(coef))
 

@@ -76,14 +76,15 @@ typedef double (*IBMSolverBCValFunc)(double, double, double, double);
 /* (t, x, y, z) => force value.  <- src/ibm3d.h:69 */
 typedef double (*IBMSolverForceFunc)(double, double, double, double);
 
-/* <- src/ibm3d.h:83-87. TURBMODEL_VREMAN is new (no reference counterpart);
-   TURBMODEL_WALE aborts in set_turb_model, as in the reference
-   (src/ibm3d_setup.c:352-353). */
+/* <- src/ibm3d.h:83-87. TURBMODEL_VREMAN and TURBMODEL_SMAGORINSKY_VD are new
+   (no reference counterpart); TURBMODEL_WALE aborts in set_turb_model, as in
+   the reference (src/ibm3d_setup.c:352-353). */
 typedef enum _turb_model_type {
     TURBMODEL_NONE,
     TURBMODEL_SMAGORINSKY,
     TURBMODEL_WALE,
     TURBMODEL_VREMAN,
+    TURBMODEL_SMAGORINSKY_VD,
 } IBMSolverTurbModelType;
 
 /* <- src/ibm3d.h:94-98 */
@@ -107,7 +108,11 @@ void IBMSolver_set_params(IBMSolver *, const double Re, const double dt);
 
 /* VAL_CONST: 3 doubles; VAL_FUNC: 3 IBMSolverForceFunc. */
 void IBMSolver_set_ext_force(IBMSolver *, IBMSolverValType, ...);
-/* TURBMODEL_SMAGORINSKY: double Cs; TURBMODEL_VREMAN: double c. */
+/* TURBMODEL_SMAGORINSKY: double Cs; TURBMODEL_VREMAN: double c;
+   TURBMODEL_SMAGORINSKY_VD (Smagorinsky with van Driest damping, l =
+   min(Cs Delta (1 - exp(-y+/A+)), 0.41 phi), y+ = y_w u_tau Re with y_w the
+   distance to the nearest outer face in `walls`): double Cs, double A+,
+   double u_tau, int walls (IBMSolverDirection bits). */
 void IBMSolver_set_turb_model(IBMSolver *, IBMSolverTurbModelType, ...);
 /* Varargs as in src/ibm3d_setup.c:378-415; values must be doubles. */
 void IBMSolver_set_bc(IBMSolver *, IBMSolverDirection, IBMSolverBCType, ...);

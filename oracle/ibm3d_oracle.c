@@ -166,6 +166,12 @@ void IBMSolver_set_turb_model(IBMSolver *s, IBMSolverTurbModelType t, ...) {
     case TURBMODEL_NONE: break;
     case TURBMODEL_SMAGORINSKY:
     case TURBMODEL_VREMAN: s->Cs = va_arg(ap, double); break;
+    case TURBMODEL_SMAGORINSKY_VD:
+        s->Cs = va_arg(ap, double);
+        s->vd_aplus = va_arg(ap, double);
+        s->vd_utau = va_arg(ap, double);
+        s->vd_walls = va_arg(ap, int);
+        break;
     default: va_end(ap); fatal("unsupported turbulence model");
     }
     va_end(ap);
@@ -892,6 +898,22 @@ static int tslot(int a, int b) {
     return t[a][b];
 }
 
+/* van Driest damping factor 1 - exp(-y+/A+), y+ = y_w u_tau Re, y_w the
+   distance of the cell centre to the nearest outer face in vd_walls (an
+   extension with no reference counterpart; van Driest, J. Aero. Sci. 23,
+   1956). */
+static double van_driest(const IBMSolver *s, int i, int j, int k) {
+    const double c[3] = {C1(s->xc, i), C1(s->yc, j), C1(s->zc, k)};
+    const double lo[3] = {s->xmin, s->ymin, s->zmin}, hi[3] = {s->xmax, s->ymax, s->zmax};
+    const int bit_lo[3] = {DIR_WEST, DIR_SOUTH, DIR_DOWN}, bit_hi[3] = {DIR_EAST, DIR_NORTH, DIR_UP};
+    double yw = 1e300;
+    for (int a = 0; a < 3; a++) {
+        if (s->vd_walls & bit_lo[a]) yw = fmin(yw, c[a] - lo[a]);
+        if (s->vd_walls & bit_hi[a]) yw = fmin(yw, hi[a] - c[a]);
+    }
+    return 1 - exp(-(yw * (s->vd_utau * s->Re)) / s->vd_aplus);
+}
+
 /* Smagorinsky  <- src/ibm3d_turb.c:8-45. Vreman (extension): nu_t =
    c sqrt(B_beta / (alpha_ij alpha_ij)) with alpha_ij = du_j/dx_i from the same
    central differences and Delta_i the cell widths (Vreman, Phys. Fluids 16,
@@ -906,13 +928,14 @@ void IBMSolver_calc_tau_r(IBMSolver *s) {
             for (int k = 0; k < Nz; k++) {
                 if (C3(s, s->flag, i, j, k) != FLAG_FLUID) continue;
                 double nut;
-                if (s->turb_type == TURBMODEL_SMAGORINSKY) {
+                if (s->turb_type == TURBMODEL_SMAGORINSKY || s->turb_type == TURBMODEL_SMAGORINSKY_VD) {
                     const double delta = 2 * cbrt(C1(s->dx, i) * C1(s->dy, j) * C1(s->dz, k));
                     double smag = 0;
                     for (int a = 1; a <= 3; a++)
                         for (int b = 1; b <= 3; b++) smag += 2 * sqd(C3(s, s->S[tslot(a, b)], i, j, k));
                     smag = sqrt(smag);
                     double l = s->Cs * delta;
+                    if (s->turb_type == TURBMODEL_SMAGORINSKY_VD && s->vd_walls) l *= van_driest(s, i, j, k);
                     const double lw = 0.41 * C3(s, s->lvset, i, j, k);
                     if (lw < l) l = lw;
                     for (int a = 1; a <= 3; a++)

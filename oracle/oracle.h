@@ -71,6 +71,8 @@ struct _ibm_solver {
 
     IBMSolverTurbModelType turb_type;
     double Cs;
+    double vd_aplus, vd_utau;  /* van Driest damping (TURBMODEL_SMAGORINSKY_VD) */
+    int vd_walls;
 
     Polyhedron *poly;
     int has_sphere;

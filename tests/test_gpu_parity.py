@@ -140,7 +140,7 @@ def test_sphere_mixed_precision(new_gpu, new_oracle, cases, I, mode):
     assert g.max_div() < 1e-8
 
 
-def setup_channel(s, I, cases, model=None, coef=0.0, n=(16, 12, 8), tol=1e-12):
+def setup_channel(s, I, cases, model=None, coef=0.0, n=(16, 12, 8), tol=1e-12, **turb):
     import math
     xf = cases.uniform_faces(0.0, 2 * math.pi, n[0])
     yf = np.array([-math.cos(math.pi * j / n[1]) for j in range(n[1] + 1)])  # cos-clustered walls
@@ -151,7 +151,7 @@ def setup_channel(s, I, cases, model=None, coef=0.0, n=(16, 12, 8), tol=1e-12):
     s.set_bc(I.DIR_NORTH | I.DIR_SOUTH, I.BC_STATIONARY_WALL)
     s.set_ext_force(1.0, 0.0, 0.0)
     if model is not None:
-        s.set_turb_model(model, coef)
+        s.set_turb_model(model, coef, **turb)
     s.set_obstacle(None)
     s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, tol)
     s.assemble()
@@ -180,6 +180,17 @@ def test_channel_les_parity(new_gpu, new_oracle, I, cases, model, coef):
     g, o = new_gpu(), new_oracle()
     for s in (g, o):
         setup_channel(s, I, cases, model=model, coef=coef)
+        s.iterate(3)
+    compare(g, o, 1e-8)
+    for n in ("TAU11", "TAU12", "TAU23"):
+        assert rel_l2(g.get(n)[INNER], o.get(n)[INNER]) < 1e-8
+
+
+def test_channel_smagorinsky_van_driest_parity(new_gpu, new_oracle, I, cases):
+    g, o = new_gpu(), new_oracle()
+    for s in (g, o):
+        setup_channel(s, I, cases, model=I.TURBMODEL_SMAGORINSKY_VD, coef=0.1, aplus=26.0, u_tau=1.0,
+                      walls=I.DIR_NORTH | I.DIR_SOUTH)
         s.iterate(3)
     compare(g, o, 1e-8)
     for n in ("TAU11", "TAU12", "TAU23"):

@@ -398,3 +398,41 @@ def test_cavity_golden_fixture(I, new_oracle, cases):
     s.iterate(5)
     for name in ("U1", "U2", "U3", "P"):
         np.testing.assert_allclose(s.get(name), g[name], rtol=0, atol=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# a3: Smagorinsky (src/ibm3d_turb.c:8-45) and the van Driest extension
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("model,cs", [(1, 0.1), (1, 1.0), (4, 0.1), (4, 1.0)])
+def test_smagorinsky_closed_form_linear_shear(I, new_oracle, cases, model, cs):
+    """u = a y: central differences are exact, S_12 = a/2, |S| = |a|, so
+    tau_12 = -2 l^2 |a| (a/2) with l = min(Cs Delta f, 0.41 phi), Delta =
+    2 (dx dy dz)^(1/3), phi = 0.5 without a body, f = 1 (Smagorinsky) or
+    1 - exp(-y+/A+) with y+ = (distance to the y walls) u_tau Re."""
+    a, Re, aplus, utau = 3.0, 50.0, 26.0, 0.8
+    xf = cases.uniform_faces(0.0, 1.0, 6)
+    yf = np.array([-math.cos(math.pi * j / 8) for j in range(9)])
+    zf = cases.uniform_faces(0.0, 0.5, 4)
+    s = new_oracle()
+
+    def bc(s):
+        s.set_bc(I.DIR_EAST | I.DIR_WEST | I.DIR_NORTH | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP,
+                 I.BC_STATIONARY_WALL)
+        s.set_turb_model(model, cs, aplus=aplus, u_tau=utau, walls=I.DIR_NORTH | I.DIR_SOUTH)
+
+    box(I, s, xf, yf, zf, Re=Re, dt=0.01, bc=bc)
+    bind_oracle_hooks(s.lib)
+    xc, yc, zc, dx, dy, dz = coords(s)
+    s.init_flow_const(0, 0, 0, 0)
+    s.set("U1", np.broadcast_to(a * yc[None, :, None], s.shape).copy())
+    s.lib.IBMSolver_calc_tau_r(s.h)
+    t12 = s.get("TAU12")[2:-2, 2:-2, 2:-2]
+    D = 2 * np.cbrt(dx[2:-2, None, None] * dy[None, 2:-2, None] * dz[None, None, 2:-2])
+    l = cs * D
+    if model == 4:
+        yw = np.minimum(yc[2:-2] - yf[0], yf[-1] - yc[2:-2])
+        l = l * (1 - np.exp(-(yw * utau * Re) / aplus))[None, :, None]
+    l = np.minimum(l, 0.41 * 0.5)
+    expect = -2 * l**2 * abs(a) * (a / 2)
+    np.testing.assert_allclose(t12, np.broadcast_to(expect, t12.shape), rtol=1e-12, atol=0)
+    assert np.allclose(s.get("TAU13")[2:-2, 2:-2, 2:-2], 0) and np.allclose(s.get("TAU11")[2:-2, 2:-2, 2:-2], 0)
