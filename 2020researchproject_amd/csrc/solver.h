@@ -53,14 +53,23 @@ struct DBuf {
         n = cnt;
         if (cnt) {
             HIP_CHECK(hipMalloc(&p, cnt * sizeof(T)));
+            // the memset runs on the legacy stream, which the solvers'
+            // non-blocking streams do not wait for: complete it here, or a
+            // busy device could apply it after kernels have written the buffer
             HIP_CHECK(hipMemset(p, 0, cnt * sizeof(T)));
+            HIP_CHECK(hipStreamSynchronize(nullptr));
             if (tl_dev_bytes) *tl_dev_bytes += cnt * sizeof(T);
         }
     }
     void upload(const T *h, size_t cnt) {
         HipGlobalLock lk(hip_global_mutex());
         if (cnt > n) alloc(cnt);
-        if (cnt) HIP_CHECK(hipMemcpy(p, h, cnt * sizeof(T), hipMemcpyHostToDevice));
+        if (cnt) {
+            // from pageable memory the call may return before the DMA lands;
+            // the solvers' non-blocking streams do not wait for it
+            HIP_CHECK(hipMemcpy(p, h, cnt * sizeof(T), hipMemcpyHostToDevice));
+            HIP_CHECK(hipStreamSynchronize(nullptr));
+        }
     }
     void upload(const std::vector<T> &h) { upload(h.data(), h.size()); }
     void free() {

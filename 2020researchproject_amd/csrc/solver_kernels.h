@@ -69,6 +69,8 @@ void launch_rbgs(hipStream_t st, const Dims &d, const SysCoef<T> &S, const Ghost
 template <typename T>
 void launch_bicg_init(hipStream_t st, const Dims &d, T *r, T *rh, T *p, T *v, T *x, const T *b);
 template <typename T>
+void launch_bicg_restart(hipStream_t st, const Dims &d, const T *r, T *rh, T *p, T *v);
+template <typename T>
 void launch_bicg_p(hipStream_t st, const Dims &d, const KScal *ks, const T *r, const T *v, T *p);
 template <typename T>
 void launch_bicg_s(hipStream_t st, const Dims &d, const KScal *ks, const T *r, const T *v, T *s);

@@ -456,6 +456,16 @@ __global__ void __launch_bounds__(256) k_bicg_init(Dims d, T *r, T *rh, T *p, T 
     r[q] = bv; rh[q] = bv; p[q] = 0; v[q] = 0; x[q] = 0;
 }
 
+// restart from the current iterate: r holds b - A x; the shadow residual
+// becomes r, the search directions are cleared
+template <typename T>
+__global__ void __launch_bounds__(256) k_bicg_restart(Dims d, const T *r, T *rh, T *p, T *v) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    rh[q] = r[q]; p[q] = 0; v[q] = 0;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_bicg_p(Dims d, const KScal *ks, const T *r, const T *v, T *p) {
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
@@ -504,6 +514,10 @@ long vec_partials(const Dims &d) { return iblocks(d); }
 template <typename T>
 void launch_bicg_init(hipStream_t st, const Dims &d, T *r, T *rh, T *p, T *v, T *x, const T *b) {
     hipLaunchKernelGGL((k_bicg_init<T>), igrid(d), dim3(64, 4), 0, st, d, r, rh, p, v, x, b);
+}
+template <typename T>
+void launch_bicg_restart(hipStream_t st, const Dims &d, const T *r, T *rh, T *p, T *v) {
+    hipLaunchKernelGGL((k_bicg_restart<T>), igrid(d), dim3(64, 4), 0, st, d, r, rh, p, v);
 }
 template <typename T>
 void launch_bicg_p(hipStream_t st, const Dims &d, const KScal *ks, const T *r, const T *v, T *p) {
@@ -1148,6 +1162,7 @@ void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, 
                                    const uint8_t *, const T *, T *);                                               \
     template void launch_rbgs<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,              \
                                  const uint8_t *, const T *, T *, int);                                            \
+    template void launch_bicg_restart<T>(hipStream_t, const Dims &, const T *, T *, T *, T *);                \
     template void launch_bicg_init<T>(hipStream_t, const Dims &, T *, T *, T *, T *, T *, const T *);              \
     template void launch_bicg_p<T>(hipStream_t, const Dims &, const KScal *, const T *, const T *, T *);           \
     template void launch_bicg_s<T>(hipStream_t, const Dims &, const KScal *, const T *, const T *, T *);           \

@@ -134,6 +134,8 @@ static OuterParams make_outer(IBMSolver *s, int what, double t, double norm2[4])
                     if (norm2)
                         for (int c = 0; c < 4; c++) norm2[c] += 2 * h[c * (size_t)n1 * n2 + o] * h[c * (size_t)n1 * n2 + o];
                 }
+            // kernels queued earlier in this step may still read the buffer
+            HIP_CHECK(hipStreamSynchronize(s->stream));
             s->bc_arr[ax][hi].upload(h);
             F.garr = s->bc_arr[ax][hi].p;
         }
@@ -480,8 +482,9 @@ template <> const LevelDev<float> *lvdev<float>(IBMSolver *s) { return s->mg_lv3
 // written on half of the cells), ghost rows relaxed after each colour.
 template <typename T>
 static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
-                   const T *b, T *x, T *xalt, int sweeps, bool zero_init, bool fine) {
+                   const T *b, T *x, T *xalt, int sweeps, bool zero_init, int level) {
     const double n = (double)d.nx * d.ny * d.nz;
+    char nm[32];
     if (sweeps == 0) {
         if (zero_init) TIMED("mg_zero", n * sizeof(T), launch_zero<T>(s->stream, d, x));
         return;
@@ -491,7 +494,8 @@ static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const Ghost
             for (int c = 0; c < 2; c++) {
                 // the first colour from a zero start writes every cell
                 const bool z = zero_init && sw == 0 && c == 0;
-                TIMED(fine ? "rbgs_fine" : "rbgs", z ? n * (1.5 * sizeof(T) + 1) : n * (2 * sizeof(T) + 1),
+                snprintf(nm, sizeof nm, "rbgs_L%d", level);
+                TIMED(nm, z ? n * (1.5 * sizeof(T) + 1) : n * (2 * sizeof(T) + 1),
                       launch_rbgs_color<T>(s->stream, d, S, flag, b, x, c, z ? 1 : 0));
                 if (G.ng) TIMED("ghost_relax", 0, launch_ghost_relax<T>(s->stream, G, S.is_p, b, x));
             }
@@ -500,7 +504,8 @@ static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const Ghost
     T *cur = x, *nxt = xalt;
     for (int sw = 0; sw < sweeps; sw++) {
         const bool z = zero_init && sw == 0;
-        TIMED(fine ? "rbgs_fused_fine" : "rbgs_fused", n * ((z ? 2 : 3) * sizeof(T) + 1),
+        snprintf(nm, sizeof nm, "rbgs_fused_L%d", level);
+        TIMED(nm, n * ((z ? 2 : 3) * sizeof(T) + 1),
               launch_rbgs_fused<T>(s->stream, d, S, flag, b, cur, nxt, z ? 1 : 0));
         if (G.ng) TIMED("ghost_relax", 0, launch_ghost_relax<T>(s->stream, G, S.is_p, b, nxt));
         std::swap(cur, nxt);
@@ -527,7 +532,7 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
                                                      L.dxyz.p));
         return;
     }
-    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, s->mg_pre, true, fine);
+    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, s->mg_pre, true, l);
     MGLevel &Cl = s->mg[l + 1];
     if (fine) {
         // fine level: residual, then restriction (measured faster than the
@@ -540,14 +545,19 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
                                  Cl.dxyz.p + (Cl.d.nx + 4) + (Cl.d.ny + 4), lev_b<T>(Cl).p));
     } else {
         // residual restricted in one pass: x and b read, one flag byte per cell
-        TIMED("mg_resid_restrict", 2 * cell_b + cell_b / sizeof(T),
+        char nm[32];
+        snprintf(nm, sizeof nm, "mg_resid_restrict_L%d", l);
+        TIMED(nm, 2 * cell_b + cell_b / sizeof(T),
               launch_resid_restrict<T>(s->stream, L.d, S, flag, b, x, Cl.d, Cl.tr, L.dxyz.p, Cl.dxyz.p,
                                        lev_b<T>(Cl).p));
     }
     vcycle<T>(s, l + 1, lev_b<T>(Cl).p, lev_x<T>(Cl).p);
-    TIMED("mg_prolong", cell_b * 2.125,
+    char pnm[32];
+    snprintf(pnm, sizeof pnm, "mg_prolong_L%d", l);
+    // fine x read and written, one flag byte per fine cell (coarse reads cached)
+    TIMED(pnm, 2 * cell_b + cell_b / sizeof(T),
           launch_prolong<T>(s->stream, L.d, Cl.d, Cl.tr, flag, Cl.flag.p, lev_x<T>(Cl).p, x));
-    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, s->mg_post, false, fine);
+    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, s->mg_post, false, l);
 }
 
 // V-cycle as a preconditioner. Without per-kernel timing the cycle for a
@@ -606,7 +616,8 @@ static void mg_precond_f32(IBMSolver *s, const double *in, double *out) {
 // ---------------------------------------------------------------------------
 template <typename T>
 static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, const T *b, T *x, int prec,
-                    const NormSpec &ns, double tol, int maxit, double bnorm, double *final_rel, const char *tag) {
+                    const NormSpec &ns, double tol, int maxit, double bnorm, double *final_rel, const char *tag,
+                    int restart = 0) {
     const Dims &d = s->d;
     DBuf<T> *V = kvec<T>(s->kry);
     T *r = V[0].p, *rh = V[1].p, *p = V[2].p, *v = V[3].p, *ph = V[4].p, *sv = V[5].p, *sh = V[6].p, *t = V[7].p;
@@ -636,6 +647,7 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         }
     };
     int it = 0;
+    for (;;) {
     while (it < maxit && rn > tol * bn) {
         it++;
         snprintf(nm, sizeof nm, "bicg_vec");
@@ -652,14 +664,39 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         launch_finalize(s->stream, 2, s->partial.p, nv, 3, s->ks.p);
         read_ks(s);
         rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
-        if (s->ks_h->breakdown || !std::isfinite(rn)) break;
+        const bool bad = s->ks_h->breakdown || !std::isfinite(rn);
+        if (restart > 0 && it < maxit && (bad || it % restart == 0) && !(rn <= tol * bn)) {
+            // Restart from the true residual of the current iterate. With a
+            // preconditioner that is not exactly the same linear map every
+            // time (an FP32 multigrid cycle), the recurrences drift from the
+            // true residual; restarting resets them, keeping x.
+            TIMED("residual", 3 * cb, launch_residual<T>(s->stream, d, S, G, flag, b, x, r, ns, s->partial.p));
+            launch_bicg_restart<T>(s->stream, d, r, rh, p, v);
+            launch_finalize(s->stream, 3, s->partial.p, na, 2, s->ks.p);
+            read_ks(s);
+            rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
+            if (!std::isfinite(rn)) break;
+            continue;
+        }
+        if (bad) break;
     }
     // true residual of the final iterate
     TIMED("residual", 3 * cb, launch_residual<T>(s->stream, d, S, G, flag, b, x, r, ns, s->partial.p));
     launch_finalize(s->stream, 4, s->partial.p, na, 2, s->ks.p);
     read_ks(s);
-    *final_rel = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr) / bn;
+    const double rt = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
+    if (restart > 0 && it < maxit && std::isfinite(rt) && rt > tol * bn) {
+        // the recurrence met the tolerance but the true residual did not:
+        // restart from it (r already holds b - A x)
+        launch_bicg_restart<T>(s->stream, d, r, rh, p, v);
+        launch_finalize(s->stream, 3, s->partial.p, na, 2, s->ks.p);
+        read_ks(s);
+        rn = rt;
+        continue;
+    }
+    *final_rel = rt / bn;
     return it;
+    }
 }
 
 // FP64 iterative refinement around FP32 BiCGSTAB inner solves.
@@ -704,6 +741,9 @@ static int refine_mixed(IBMSolver *s, int sk, const double *b, double *x, int pr
     return total;
 }
 
+// BiCGSTAB iterations between restarts when the preconditioner runs in FP32
+static const int kMixedRestart = 20;
+
 static void solve_system(IBMSolver *s, int sk, const double *b, double *x, double bnorm) {
     const int prec = sk == 3 ? (s->precond_type == PRECOND_AMG ? 2 : 1) : 1;
     NormSpec ns{0, nullptr, nullptr, nullptr, nullptr, -1};
@@ -716,7 +756,7 @@ static void solve_system(IBMSolver *s, int sk, const double *b, double *x, doubl
     } else if (s->precision == IBM3D_PREC_MIXED_KRYLOV) {
         const GhostData &GD = sk == 3 ? s->gp : s->gmom;
         it = bicgstab<double>(s, s->sys[sk].S64, GD.G64, b, x, prec == 2 ? 3 : prec, ns, s->tol, maxit, bnorm, &rel,
-                              sk == 3 ? "p64" : "u64");
+                              sk == 3 ? "p64" : "u64", prec == 2 ? kMixedRestart : 0);
     } else {
         const GhostData &GD = sk == 3 ? s->gp : s->gmom;
         it = bicgstab<double>(s, s->sys[sk].S64, GD.G64, b, x, prec, ns, s->tol, maxit, bnorm, &rel,
@@ -751,6 +791,7 @@ static StepParams step_params(IBMSolver *s) {
                 for (int j = 0; j < s->Ny; j++)
                     for (int i = 0; i < s->Nx; i++)
                         h[cidx(s->d, i, j, k)] = s->func_f[l](s->time, H1(s->xc, i), H1(s->yc, j), H1(s->zc, k));
+            HIP_CHECK(hipStreamSynchronize(s->stream));  // previous readers of the buffer
             s->force_arr[l].upload(h);
             sp.fa[l] = s->force_arr[l].p;
         }
