@@ -109,6 +109,18 @@ def cpu_baseline(tol: float, steps: int = 3):
                       f"FP64 oracle, {dt:.2f} s"}
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes recorded in profiles/traffic.json (each counter scaled
+    by its reading on a plain device copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f).get("kernels", {}).get(kernel)
+    return None if k is None else k["traffic_bytes"]
+
+
 def roofline(stats):
     """Roofline entry of the streaming kernel with the largest total time
     (kernels with a stated algorithmic byte count per launch)."""
@@ -119,9 +131,9 @@ def roofline(stats):
     avg_ms = dom["ms"] / dom["count"]
     achieved = dom["bytes"] / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": dom["name"], "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes"],
-            "launches": dom["count"]}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom["name"]),
+            "kernel": dom["name"], "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": round(dom["bytes"]),
+            "launches": dom["count"], "traffic_source": "profiles/traffic.json"}
 
 
 def top_kernels(stats, n: int = 8):
