@@ -53,9 +53,9 @@ IBMSolver *IBMSolver_new(const int num_process, const int rank, const int Px, co
     DevBytesScope account(&s->dev_bytes);
     HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_CHECK(hipHostMalloc((void **)&s->ks_h, sizeof(KScal), hipHostMallocDefault));
-    HIP_CHECK(hipHostMalloc((void **)&s->sums_h, 16 * sizeof(double), hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc((void **)&s->sums_h, kSumSlots * sizeof(double), hipHostMallocDefault));
     s->ks.alloc(1);
-    s->sums.alloc(16);
+    s->sums.alloc(kSumSlots);
     return s;
 }
 
@@ -156,6 +156,10 @@ void IBMSolver_set_bc(IBMSolver *s, IBMSolverDirection dir, IBMSolverBCType type
     case BC_SYMMETRIC:
         type = BC_FREE_SLIP_WALL;
         break;
+    case BC_CONVECTIVE_OUTFLOW:
+        v.uc = va_arg(ap, double);
+        v.val_type = VAL_CONST;
+        break;
     default:;
     }
     va_end(ap);
@@ -205,6 +209,12 @@ void IBMSolver_set_mg_params(IBMSolver *s, double gamma, int pre, int post, int 
     s->mg_pre = pre;
     s->mg_post = post;
     s->mg_smoother = smoother;
+}
+
+void IBMSolver_set_mg_light_levels(IBMSolver *s, int from_level) { s->mg_light_from = from_level; }
+void IBMSolver_set_mixed_params(IBMSolver *s, double inner_tol, int momentum_fp32) {
+    s->mixed_inner_tol = inner_tol;
+    s->mixed_momentum_fp32 = momentum_fp32 != 0;
 }
 
 // Mirror point and trilinear weights  <- src/utils.c:111-171 (base indices
@@ -450,7 +460,8 @@ void build_system(const IBMSolver *s, const std::vector<double> *D[3], const std
             const int o1 = hi ? N : -1, q0 = hi ? N - 1 : 0, q1 = hi ? N - 2 : 1;
             const double ao = H1(dd, o1) / 2, bi = H1(dd, q0) / 2;
             double alpha = 1, beta = 0, gamma = 0;
-            const bool dirichlet_u = t == BC_VELOCITY_COMPONENT || t == BC_STATIONARY_WALL;
+            const bool dirichlet_u = t == BC_VELOCITY_COMPONENT || t == BC_STATIONARY_WALL
+                                  || t == BC_CONVECTIVE_OUTFLOW;
             if (sk < 3) {
                 if (dirichlet_u) { alpha = -ao / bi; gamma = (ao + bi) / bi; }
                 else if (t == BC_PRESSURE) {
@@ -462,7 +473,10 @@ void build_system(const IBMSolver *s, const std::vector<double> *D[3], const std
                     alpha = (sk == a) ? -ao / bi : 1;
                 }
             } else {
-                if (t == BC_PRESSURE || t == BC_VELOCITY_PERIODIC) { alpha = -ao / bi; gamma = (ao + bi) / bi; }
+                if (t == BC_PRESSURE || t == BC_VELOCITY_PERIODIC || t == BC_CONVECTIVE_OUTFLOW) {
+                    alpha = -ao / bi;
+                    gamma = (ao + bi) / bi;
+                }
                 else alpha = 1;  // wall, velocity, free slip: Neumann (1, -1) rows
             }
             out.gamma[a][hi] = gamma;
@@ -558,7 +572,9 @@ extern "C" void IBMSolver_assemble(IBMSolver *s) {
     }
     s->p_singular = true;  // <- src/ibm3d_setup.c:1598-1603
     for (int i = 0; i < 6; i++)
-        if (s->bc[i].type == BC_PRESSURE || s->bc[i].type == BC_VELOCITY_PERIODIC) s->p_singular = false;
+        if (s->bc[i].type == BC_PRESSURE || s->bc[i].type == BC_VELOCITY_PERIODIC
+            || s->bc[i].type == BC_CONVECTIVE_OUTFLOW)
+            s->p_singular = false;
 
     calc_geometry(s);
     calc_lvset_flag(s);
@@ -607,6 +623,13 @@ extern "C" void IBMSolver_assemble(IBMSolver *s) {
     bc_rows_norm(s);
 
     alloc_fields(s);
+    {
+        const long ext[3] = {(long)s->Ny * s->Nz, (long)(s->Nx + 4) * s->Nz, (long)(s->Nx + 4) * (s->Ny + 4)};
+        for (int a = 0; a < 3; a++)
+            for (int hi = 0; hi < 2; hi++)
+                if (s->bc[face_bc(a, hi)].type == BC_CONVECTIVE_OUTFLOW) s->conv_g[face_bc(a, hi)].alloc(4 * ext[a]);
+        s->conv_ready = false;
+    }
     build_mg(s);
     s->assembled = true;
     HIP_CHECK(hipStreamSynchronize(s->stream));
@@ -628,6 +651,7 @@ extern "C" void IBMSolver_init_flow_const(IBMSolver *s, const double u1, const d
     if (!s->assembled) fatal("IBMSolver_assemble must precede IBMSolver_init_flow_const");
     s->iter = 0;
     s->time = 0;
+    s->conv_ready = false;
     const double v[4] = {u1, u2, u3, p};
     DBuf<double> *dst[4] = {&s->u[0], &s->u[1], &s->u[2], &s->p};
     std::vector<double> h(s->d.ncell);
@@ -640,6 +664,7 @@ extern "C" void IBMSolver_init_flow_const(IBMSolver *s, const double u1, const d
 extern "C" void IBMSolver_init_flow_func(IBMSolver *s, IBMSolverInitFunc f1, IBMSolverInitFunc f2,
                                          IBMSolverInitFunc f3, IBMSolverInitFunc fp) {
     if (!s->assembled) fatal("IBMSolver_assemble must precede IBMSolver_init_flow_func");
+    s->conv_ready = false;
     IBMSolverInitFunc f[4] = {f1, f2, f3, fp};
     DBuf<double> *dst[4] = {&s->u[0], &s->u[1], &s->u[2], &s->p};
     const long P = (long)(s->Nx + 4) * (s->Ny + 4) * (s->Nz + 4);
@@ -726,6 +751,7 @@ extern "C" void IBMSolver_set_field(IBMSolver *s, int field, const double *in) {
         fatal("set_field supports u1, u2, u3, p");
     HIP_CHECK(hipStreamSynchronize(s->stream));
     upload_cell_field(s, *cell_field(s, field), in);
+    s->conv_ready = false;
 }
 
 extern "C" void IBMSolver_get_flag(IBMSolver *s, int *out) {
@@ -746,6 +772,16 @@ extern "C" int IBMSolver_get_ghost_cells(IBMSolver *s, int *out) {
                     if (out) { out[3 * n] = i; out[3 * n + 1] = j; out[3 * n + 2] = k; }
                     n++;
                 }
+    return n;
+}
+
+extern "C" int IBMSolver_get_mg_levels(IBMSolver *s, int *out, int max) {
+    const int n = (int)s->mg.size();
+    for (int l = 0; l < n && l < max && out; l++) {
+        out[3 * l] = s->mg[l].d.nx;
+        out[3 * l + 1] = s->mg[l].d.ny;
+        out[3 * l + 2] = s->mg[l].d.nz;
+    }
     return n;
 }
 

@@ -324,6 +324,7 @@ void export_intermediate_impl(IBMSolver *s, const char *filename) {
 // <- src/ibm3d_init.c:45-224: time, iter, u, v, w, p from "<filename>.nc"
 void init_flow_file_impl(IBMSolver *s, const char *filename) {
     if (!s->assembled) fatal("IBMSolver_assemble must precede IBMSolver_init_flow_file");
+    s->conv_ready = false;
     NcFile nc;
     FILE *f = fopen(with_ext(filename).c_str(), "rb");
     if (!f) {

@@ -86,6 +86,7 @@ struct HostBC {
     IBMSolverValType val_type = VAL_CONST;
     double cu[3] = {0, 0, 0}, cp = 0;
     IBMSolverBCValFunc fu[3] = {nullptr, nullptr, nullptr}, fp = nullptr;
+    double uc = 0;  // BC_CONVECTIVE_OUTFLOW speed (<= 0: mean outward velocity)
     bool set = false;
 };
 
@@ -187,6 +188,9 @@ struct _ibm_solver {
     double mg_gamma = 3.0;       // coarsening threshold (multiples of the level's smallest width)
     int mg_pre = 2, mg_post = 2; // smoothing sweeps per level
     int mg_smoother = 1;         // 0 fused red-black sweeps, 1 one launch per colour
+    int mg_light_from = 0;       // levels >= this (if > 0) smooth with one sweep before and after
+    double mixed_inner_tol = 1e-4;  // IBM3D_PREC_MIXED: relative tolerance of the FP32 inner solves
+    bool mixed_momentum_fp32 = true;  // IBM3D_PREC_MIXED: momentum solves refined in FP32 too
 
     // ---- host mirrors ----
     std::vector<double> lvset_h;     // reference layout (k fastest)
@@ -206,7 +210,9 @@ struct _ibm_solver {
     ibm::DBuf<double> tau[6];
     ibm::DBuf<double> force_arr[3];
     ibm::DBuf<double> bc_arr[3][2];  // host-evaluated BC data of VAL_FUNC faces
-    ibm::DBuf<double> partial, sums;
+    ibm::DBuf<double> conv_g[6];     // BC_CONVECTIVE_OUTFLOW face values (4 comps, bc[] order)
+    bool conv_ready = false;         // conv_g hold the current face values
+    ibm::DBuf<double> partial, sums;  // sums: kSumSlots doubles
     ibm::DBuf<ibm::KScal> ks;
     ibm::KScal *ks_h = nullptr;      // pinned
     double *sums_h = nullptr;        // pinned
@@ -241,6 +247,11 @@ struct _ibm_solver {
 };
 
 namespace ibm {
+
+// slots of IBMSolver::sums: 0..11 reductions of the step kernels, 12..29 the
+// convective outflow faces' BC-row norms (3 per face, bc[] order)
+constexpr int kSumSlots = 32;
+constexpr int kConvNormSlot = 12;
 
 // api.cpp
 long host_ref_index(const IBMSolver *s, int i, int j, int k);

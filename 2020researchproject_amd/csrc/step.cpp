@@ -114,6 +114,12 @@ static OuterParams make_outer(IBMSolver *s, int what, double t, double norm2[4])
             for (int c = 0; c < 3; c++) F.g[c] = B.cu[c];
             F.g[3] = what == BCD_PSOL ? 0.0 : B.cp;
             F.garr = nullptr;
+            if (B.type == BC_CONVECTIVE_OUTFLOW) {
+                // device-resident face values (advanced by conv_outflow), p = 0
+                for (int c = 0; c < 4; c++) F.g[c] = 0;
+                F.garr = s->conv_g[face_bc(ax, hi)].p;
+                continue;
+            }
             if (B.val_type != VAL_FUNC) continue;
             std::vector<double> h(4 * (size_t)n1 * n2, 0.0);
             for (int a = 0; a < n1; a++)
@@ -140,6 +146,24 @@ static OuterParams make_outer(IBMSolver *s, int what, double t, double norm2[4])
             F.garr = s->bc_arr[ax][hi].p;
         }
     return op;
+}
+
+// Convective outflow faces: init copies the first interior cell's velocity,
+// otherwise one upwind step of the face values from the current u; each face
+// leaves its BC-row norms in sums[kConvNormSlot + 3 bc + c]. Returns whether
+// any face has this condition.
+static bool conv_outflow(IBMSolver *s, int init) {
+    bool any = false;
+    for (int ax = 0; ax < 3; ax++)
+        for (int hi = 0; hi < 2; hi++) {
+            const int bci = face_bc(ax, hi);
+            if (s->bc[bci].type != BC_CONVECTIVE_OUTFLOW) continue;
+            any = true;
+            TIMED("conv_outflow", 0,
+                  launch_conv_outflow(s->stream, s->geo, ax, hi, s->bc[bci].uc, s->dt, init, s->u[0].p, s->u[1].p,
+                                      s->u[2].p, s->conv_g[bci].p, s->sums.p + kConvNormSlot + 3 * bci));
+        }
+    return any;
 }
 
 static void update_outer(IBMSolver *s) {
@@ -532,7 +556,8 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
                                                      L.dxyz.p));
         return;
     }
-    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, s->mg_pre, true, l);
+    const bool light = s->mg_light_from > 0 && l >= s->mg_light_from;
+    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, light ? 1 : s->mg_pre, true, l);
     MGLevel &Cl = s->mg[l + 1];
     if (fine) {
         // fine level: residual, then restriction (measured faster than the
@@ -557,7 +582,7 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
     // fine x read and written, one flag byte per fine cell (coarse reads cached)
     TIMED(pnm, 2 * cell_b + cell_b / sizeof(T),
           launch_prolong<T>(s->stream, L.d, Cl.d, Cl.tr, flag, Cl.flag.p, lev_x<T>(Cl).p, x));
-    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, s->mg_post, false, l);
+    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, light ? 1 : s->mg_post, false, l);
 }
 
 // V-cycle as a preconditioner. Without per-kernel timing the cycle for a
@@ -720,7 +745,7 @@ static int refine_mixed(IBMSolver *s, int sk, const double *b, double *x, int pr
     double rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
     int total = 0, outer = 0;
     const NormSpec off{0, nullptr, nullptr, nullptr, nullptr, -1};
-    const double inner_tol = 1e-4;
+    const double inner_tol = s->mixed_inner_tol;
     while (rn > tol * bn && outer < 20 && total < maxit) {
         outer++;
         TIMED("d2f", cb * 12, launch_d2f(s->stream, d, r64, rhs32));
@@ -751,7 +776,7 @@ static void solve_system(IBMSolver *s, int sk, const double *b, double *x, doubl
     const int maxit = sk == 3 ? 1000 : 200;  // src/ibm3d_setup.c:1413, 1467
     double rel = 0;
     int it;
-    if (s->precision == IBM3D_PREC_MIXED) {
+    if (s->precision == IBM3D_PREC_MIXED && (sk == 3 || s->mixed_momentum_fp32)) {
         it = refine_mixed(s, sk, b, x, prec, ns, s->tol, maxit, bnorm, &rel);
     } else if (s->precision == IBM3D_PREC_MIXED_KRYLOV) {
         const GhostData &GD = sk == 3 ? s->gp : s->gmom;
@@ -841,20 +866,26 @@ static void do_step(IBMSolver *s) {
                               s->partial.p));
         launch_sum_partials(s->stream, s->partial.p, conv_rhs_blocks(s->d), 3, s->sums.p, 0);
     }
+    const bool conv = conv_outflow(s, 0);
     double fnorm[4];
     const OuterParams ob = make_outer(s, BCD_USOL, s->time, fnorm);
     for (int l = 0; l < 3; l++)
         for (int ax = 0; ax < 3; ax++)
             for (int hi = 0; hi < 2; hi++) {
                 const double gm = s->sys[l].gamma[ax][hi];
-                if (gm == 0 || s->bc[face_bc(ax, hi)].type != BC_VELOCITY_COMPONENT) continue;
+                const IBMSolverBCType t = s->bc[face_bc(ax, hi)].type;
+                if (gm == 0 || (t != BC_VELOCITY_COMPONENT && t != BC_CONVECTIVE_OUTFLOW)) continue;
                 const double *Kf = ax == 0 ? (hi ? s->g1d[7].p : s->g1d[6].p)
                                  : ax == 1 ? (hi ? s->g1d[9].p : s->g1d[8].p) : (hi ? s->g1d[11].p : s->g1d[10].p);
                 launch_bc_rhs(s->stream, s->geo, ob, ax, hi, l, Kf, gm, s->b[l].p);
             }
-    read_sums(s, 3);
+    read_sums(s, conv ? kConvNormSlot + 18 : 3);
     double bn2[3];
-    for (int l = 0; l < 3; l++) bn2[l] = s->sums_h[l] + s->sys[l].bc_norm2 + fnorm[l];
+    for (int l = 0; l < 3; l++) {
+        bn2[l] = s->sums_h[l] + s->sys[l].bc_norm2 + fnorm[l];
+        for (int f = 0; conv && f < 6; f++)
+            if (s->bc[f].type == BC_CONVECTIVE_OUTFLOW) bn2[l] += s->sums_h[kConvNormSlot + 3 * f + l];
+    }
     // momentum Helmholtz solves
     for (int l = 0; l < 3; l++) {
         solve_system(s, l, s->b[l].p, s->us[l].p, std::sqrt(bn2[l]));
@@ -904,6 +935,8 @@ static void do_step(IBMSolver *s) {
 
 void iterate_impl(IBMSolver *s, int steps, bool verbose) {
     const double t0 = s->time;
+    if (!s->conv_ready) conv_outflow(s, 1);
+    s->conv_ready = true;
     // prologue  <- src/ibm3d_fracstep.c:46-54
     update_outer(s);
     launch_interp_stag(s->stream, s->geo, s->u[0].p, s->u[1].p, s->u[2].p, s->F[0].p, s->F[1].p, s->F[2].p);

@@ -35,7 +35,7 @@ struct TauPtrs {
 
 enum BCType : int {
     BCT_VELOCITY = 0, BCT_PRESSURE = 1, BCT_STATIONARY = 2, BCT_FREE_SLIP = 3,
-    BCT_SYMMETRIC = 4, BCT_ALL_PERIODIC = 5, BCT_VELOCITY_PERIODIC = 6
+    BCT_SYMMETRIC = 4, BCT_ALL_PERIODIC = 5, BCT_VELOCITY_PERIODIC = 6, BCT_CONVECTIVE = 7
 };
 
 // Boundary data of one outer face. g[0..2] velocity components, g[3]
@@ -51,6 +51,14 @@ struct FaceBC {
 struct OuterParams {
     FaceBC face[3][2];  // [axis][low/high]
 };
+
+// Convective outflow face values of one face (bci = N,E,S,W,D,U index):
+// init != 0 copies the first interior cell's velocity; otherwise one upwind
+// step of dg/dt + Uc dg/dn = 0. garr holds 4 components x the face's
+// tangential extent (the 4th, the pressure value, stays 0); norm_out[0..2]
+// receives 2 sum g_c^2 over the face (the BC rows' share of |b|^2).
+void launch_conv_outflow(hipStream_t st, const Geo &g, int ax, int hi, double uc, double dt, int init,
+                         const double *u1, const double *u2, const double *u3, double *garr, double *norm_out);
 
 void launch_interp_stag(hipStream_t st, const Geo &g, const double *u1, const double *u2, const double *u3,
                         double *U1, double *U2, double *U3);

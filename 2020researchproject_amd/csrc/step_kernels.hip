@@ -318,7 +318,7 @@ __global__ void k_outer(Geo g, OuterParams op, int ax, double *f1, double *f2, d
         const int o = layer == 1 ? o1 : o2, qq = layer == 1 ? q0 : q1;
         const long co = base + o * st, cq = base + qq * st;
         // Dirichlet interpolation distances (ao outer, bi inner)
-        const double *dl = (mode == 1 && ax == 2 && !hi) ? g.dx : dd;  // :1792-1800 spacing quirk
+        const double *dl = (mode == 1 && ax == 2 && !hi && F.type != BCT_CONVECTIVE) ? g.dx : dd;  // :1792-1800 spacing quirk
         double ao, bi;
         if (layer == 1) { ao = C1(dl, o1) / 2; bi = C1(dl, q0) / 2; }
         else { ao = C1(dl, o2) / 2 + C1(dl, o1); bi = C1(dl, q0) + C1(dl, q1) / 2; }
@@ -342,7 +342,8 @@ __global__ void k_outer(Geo g, OuterParams op, int ax, double *f1, double *f2, d
             for (int l = l0; l < l1; l++) {
                 double *a = ff[mode == 0 ? l : 0];
                 switch (F.type) {
-                case BCT_VELOCITY: {
+                case BCT_VELOCITY:
+                case BCT_CONVECTIVE: {
                     const double gv = bcval(F, l, t1, t2, n2);
                     a[co] = (ao + bi) / bi * gv - ao / bi * a[cq];
                     break;
@@ -368,7 +369,8 @@ __global__ void k_outer(Geo g, OuterParams op, int ax, double *f1, double *f2, d
                 a[co] = a[cq];
                 break;
             case BCT_PRESSURE:
-            case BCT_VELOCITY_PERIODIC: {
+            case BCT_VELOCITY_PERIODIC:
+            case BCT_CONVECTIVE: {
                 const double gv = bcval(F, 3, t1, t2, n2);
                 a[co] = (ao + bi) / bi * gv - ao / bi * a[cq];
                 break;
@@ -380,6 +382,84 @@ __global__ void k_outer(Geo g, OuterParams op, int ax, double *f1, double *f2, d
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Convective outflow (extension; no counterpart in the reference). One
+// workgroup per face, so the face sums below have one fixed order.
+// ---------------------------------------------------------------------------
+static constexpr int kConvThreads = 1024;
+
+// fixed-order tree sum over the workgroup; every thread gets the result
+__device__ inline double conv_block_sum(double v, double *red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kConvThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kConvThreads)
+k_conv_outflow(Geo g, int ax, int hi, double uc, double dt, int init, const double *u1, const double *u2,
+               const double *u3, double *garr, double *norm_out) {
+    __shared__ double red[kConvThreads];
+    const Dims &d = g.d;
+    int a0, a1, b0, b1;
+    if (ax == 0) { a0 = 0; a1 = d.ny; b0 = 0; b1 = d.nz; }
+    else if (ax == 1) { a0 = -2; a1 = d.nx + 2; b0 = 0; b1 = d.nz; }
+    else { a0 = -2; a1 = d.nx + 2; b0 = -2; b1 = d.ny + 2; }
+    const int n2 = b1 - b0;
+    const long ext = (long)(a1 - a0) * n2;
+    const int N = ax == 0 ? d.nx : ax == 1 ? d.ny : d.nz;
+    const int q0 = hi ? N - 1 : 0;
+    const long st = ax == 0 ? 1 : ax == 1 ? d.sy : d.sz;
+    const double bi = C1(ax == 0 ? g.dx : ax == 1 ? g.dy : g.dz, q0) / 2;
+    if (!init && uc <= 0) {
+        // area-weighted mean outward normal component over the interior cells
+        const int m1 = ax == 0 ? d.ny : d.nx, m2 = ax == 2 ? d.ny : d.nz;
+        const double *w1 = ax == 0 ? g.dy : g.dx, *w2 = ax == 2 ? g.dy : g.dz;
+        double num = 0, den = 0;
+        for (long e = threadIdx.x; e < (long)m1 * m2; e += kConvThreads) {
+            const int t1 = (int)(e / m2), t2 = (int)(e % m2);
+            const double A = C1(w1, t1) * C1(w2, t2);
+            num += garr[ax * ext + (long)(t1 - a0) * n2 + (t2 - b0)] * A;
+            den += A;
+        }
+        num = conv_block_sum(num, red);
+        den = conv_block_sum(den, red);
+        uc = fmax((hi ? num : -num) / den, 0.0);
+    }
+    const double *u[3] = {u1, u2, u3};
+    double nrm[3] = {0, 0, 0};
+    for (long e = threadIdx.x; e < ext; e += kConvThreads) {
+        const int t1 = a0 + (int)(e / n2), t2 = b0 + (int)(e % n2);
+        long base;
+        if (ax == 0) base = cidx(d, 0, t1, t2);
+        else if (ax == 1) base = cidx(d, t1, 0, t2);
+        else base = cidx(d, t1, t2, 0);
+        const long cq = base + q0 * st;
+        for (int c = 0; c < 3; c++) {
+            const double uq = u[c][cq];
+            double gv = garr[c * ext + e];
+            gv = init ? uq : gv - dt * uc * (gv - uq) / bi;
+            garr[c * ext + e] = gv;
+            nrm[c] += 2 * gv * gv;
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        const double t = conv_block_sum(nrm[c], red);
+        if (threadIdx.x == 0) norm_out[c] = t;
+    }
+}
+
+void launch_conv_outflow(hipStream_t st, const Geo &g, int ax, int hi, double uc, double dt, int init,
+                         const double *u1, const double *u2, const double *u3, double *garr, double *norm_out) {
+    hipLaunchKernelGGL(k_conv_outflow, dim3(1), dim3(kConvThreads), 0, st, g, ax, hi, uc, dt, init, u1, u2, u3,
+                       garr, norm_out);
 }
 
 // ---------------------------------------------------------------------------

@@ -24,6 +24,7 @@ PRECOND_NONE, PRECOND_AMG = range(2)
 DIR_NORTH, DIR_EAST, DIR_SOUTH, DIR_WEST, DIR_DOWN, DIR_UP = (1 << i for i in range(6))
 (BC_VELOCITY_COMPONENT, BC_PRESSURE, BC_STATIONARY_WALL, BC_FREE_SLIP_WALL,
  BC_SYMMETRIC, BC_ALL_PERIODIC, BC_VELOCITY_PERIODIC) = range(7)
+BC_CONVECTIVE_OUTFLOW = 7  # extension: convective outflow, varargs (double Uc)
 VAL_CONST, VAL_FUNC = range(2)
 TURBMODEL_NONE, TURBMODEL_SMAGORINSKY, TURBMODEL_WALE, TURBMODEL_VREMAN, TURBMODEL_SMAGORINSKY_VD = range(5)
 FLAG_FLUID, FLAG_GHOST, FLAG_SOLID = 1, 2, 3
@@ -88,6 +89,7 @@ _OPTIONAL = {
     "IBMSolver_get_kernel_stat": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, _dp,
                                             C.POINTER(C.c_long), _dp]),
     "IBMSolver_get_device_bytes": (C.c_double, [_vp]),
+    "IBMSolver_get_mg_levels": (C.c_int, [_vp, _ip, C.c_int]),
     "IBMSolver_estimate_device_bytes": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "IBMServer_new": (_vp, [C.c_double, C.c_int]),
     "IBMServer_submit": (C.c_int, [_vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, _vp]),
@@ -146,10 +148,17 @@ class Solver:
         self.lib.IBMSolver_set_grid(self.h, self.Nx, self.Ny, self.Nz, _dptr(xf), _dptr(yf), _dptr(zf))
 
     def set_params(self, Re: float, dt: float):
+        self.Re, self.dt = float(Re), float(dt)
         self.lib.IBMSolver_set_params(self.h, float(Re), float(dt))
 
     def set_bc(self, direction: int, bctype: int, *vals):
-        """vals as in src/ibm3d_setup.c:378-415, e.g. (VAL_CONST, 1., 0., 0.)."""
+        """vals as in src/ibm3d_setup.c:378-415, e.g. (VAL_CONST, 1., 0., 0.);
+        BC_CONVECTIVE_OUTFLOW takes (Uc,), Uc <= 0 for the mean outward
+        velocity."""
+        if bctype == BC_CONVECTIVE_OUTFLOW:
+            uc = float(vals[0]) if vals else 0.0
+            self.lib.IBMSolver_set_bc(C.c_void_p(self.h), C.c_int(direction), C.c_int(bctype), C.c_double(uc))
+            return
         args = []
         for v in vals:
             if isinstance(v, int) and not isinstance(v, bool) and not args:
@@ -326,6 +335,12 @@ class Solver:
             out.append({"name": name.value.decode(), "ms": ms.value, "count": cnt.value,
                         "bytes": nbytes.value})
         return out
+
+    def mg_levels(self) -> list[tuple[int, int, int]]:
+        n = self.lib.IBMSolver_get_mg_levels(self.h, None, 0)
+        buf = (C.c_int * (3 * max(n, 1)))()
+        self.lib.IBMSolver_get_mg_levels(self.h, buf, n)
+        return [tuple(buf[3 * l:3 * l + 3]) for l in range(n)]
 
     def device_bytes(self) -> float:
         return self.lib.IBMSolver_get_device_bytes(self.h)

@@ -63,6 +63,12 @@ typedef enum _bc_type {
     BC_SYMMETRIC,
     BC_ALL_PERIODIC,
     BC_VELOCITY_PERIODIC,
+    /* Extension (no counterpart in the reference): convective outflow. The
+       velocity on the face obeys du/dt + Uc du/dn = 0 (first-order upwind
+       between the face and the first interior cell, advanced once per step
+       before the momentum solve); the pressure on the face is held at 0
+       (Dirichlet rows, as BC_PRESSURE with value 0). */
+    BC_CONVECTIVE_OUTFLOW,
 } IBMSolverBCType;
 
 /* <- src/ibm3d.h:47-50 */
@@ -114,7 +120,11 @@ void IBMSolver_set_ext_force(IBMSolver *, IBMSolverValType, ...);
    distance to the nearest outer face in `walls`): double Cs, double A+,
    double u_tau, int walls (IBMSolverDirection bits). */
 void IBMSolver_set_turb_model(IBMSolver *, IBMSolverTurbModelType, ...);
-/* Varargs as in src/ibm3d_setup.c:378-415; values must be doubles. */
+/* Varargs as in src/ibm3d_setup.c:378-415; values must be doubles.
+   BC_CONVECTIVE_OUTFLOW: double Uc, the convection speed; Uc <= 0 takes the
+   mean outward normal velocity over the face at each step. The face values
+   start from the first interior cell's velocity (zero gradient) at the first
+   IBMSolver_iterate after an init_flow_* or set_field call. */
 void IBMSolver_set_bc(IBMSolver *, IBMSolverDirection, IBMSolverBCType, ...);
 /* Borrowed; used only inside IBMSolver_assemble. NULL = no body. */
 void IBMSolver_set_obstacle(IBMSolver *, Polyhedron *);

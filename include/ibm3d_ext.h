@@ -78,6 +78,11 @@ void IBMSolver_get_last_iters(IBMSolver *, int out[4]);
    fastest): writes 3*count ints when out != NULL, returns count. */
 int IBMSolver_get_ghost_cells(IBMSolver *, int *out);
 
+/* Shapes of the pressure multigrid levels built by IBMSolver_assemble
+   (level 0 = the fine grid): writes nx, ny, nz of up to max levels into out
+   when out != NULL, returns the number of levels. */
+int IBMSolver_get_mg_levels(IBMSolver *, int *out, int max);
+
 /* max |div U| over fluid cells, U the face velocity after projection. */
 double IBMSolver_max_div(IBMSolver *);
 

@@ -69,6 +69,7 @@ static void free_mat(EllMat *A) {
 
 void IBMSolver_destroy(IBMSolver *s) {
     if (!s) return;
+    for (int i = 0; i < 6; i++) free(s->conv_g[i]);
     double **arrs[] = {
         &s->dx, &s->dy, &s->dz, &s->xc, &s->yc, &s->zc,
         &s->kx_W, &s->kx_E, &s->ky_S, &s->ky_N, &s->kz_D, &s->kz_U,
@@ -182,6 +183,7 @@ void IBMSolver_set_bc(IBMSolver *s, IBMSolverDirection dir, IBMSolverBCType type
     IBMSolverValType vt = VAL_CONST;
     double cu1 = NAN, cu2 = NAN, cu3 = NAN, cp = NAN;
     IBMSolverBCValFunc fu1 = NULL, fu2 = NULL, fu3 = NULL, fp = NULL;
+    double uc = 0;
     va_list ap;
     va_start(ap, type);
     switch (type) {
@@ -208,12 +210,17 @@ void IBMSolver_set_bc(IBMSolver *s, IBMSolverDirection dir, IBMSolverBCType type
     case BC_SYMMETRIC:
         type = BC_FREE_SLIP_WALL;
         break;
+    case BC_CONVECTIVE_OUTFLOW:
+        uc = va_arg(ap, double);
+        cp = 0;
+        break;
     default:;
     }
     va_end(ap);
     for (int i = 0; i < 6; i++) {
         if (dir & (1 << i)) {
             OBC *b = &s->bc[i];
+            s->conv_uc[i] = uc;
             b->type = type; b->val_type = vt;
             b->const_u1 = cu1; b->const_u2 = cu2; b->const_u3 = cu3; b->const_p = cp;
             b->func_u1 = fu1; b->func_u2 = fu2; b->func_u3 = fu3; b->func_p = fp;
@@ -332,6 +339,67 @@ static FacePos face_pos(int n, int hi) {
 static void dirichlet_dist(const double *d, FacePos f, int layer, double *ao, double *bi) {
     if (layer == 1) { *ao = C1(d, f.o1) / 2; *bi = C1(d, f.q0) / 2; }
     else { *ao = C1(d, f.o2) / 2 + C1(d, f.o1); *bi = C1(d, f.q0) + C1(d, f.q1) / 2; }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Convective outflow (extension; no counterpart in the reference)           */
+/* ------------------------------------------------------------------------ */
+
+static size_t conv_extent(const IBMSolver *s, int ax) {
+    int a0, a1, b0, b1;
+    face_ranges(s, ax, &a0, &a1, &b0, &b1);
+    return (size_t)(a1 - a0) * (b1 - b0);
+}
+
+/* face value of component l (1..3) at tangential position (t1, t2) */
+static double conv_val(const IBMSolver *s, int bci, int l, int t1, int t2) {
+    const int ax = bci == 1 || bci == 3 ? 0 : bci == 0 || bci == 2 ? 1 : 2;
+    int a0, a1, b0, b1;
+    face_ranges(s, ax, &a0, &a1, &b0, &b1);
+    return s->conv_g[bci][(size_t)(l - 1) * conv_extent(s, ax) + (size_t)(t1 - a0) * (b1 - b0) + (t2 - b0)];
+}
+
+/* init: g <- u at the first interior cell (zero normal gradient). Otherwise
+   one explicit upwind step of dg/dt + Uc dg/dn = 0 between the first
+   interior cell centre and the face: g <- g - dt Uc (g - u_q0) / (d_q0 / 2),
+   with Uc the set speed, or (Uc <= 0) the area-weighted mean outward normal
+   component of g over the face's interior cells, clamped at 0. */
+static void conv_outflow(IBMSolver *s, int init) {
+    const double *u[4] = {NULL, s->u1, s->u2, s->u3};
+    for (int bci = 0; bci < 6; bci++) {
+        if (s->bc[bci].type != BC_CONVECTIVE_OUTFLOW) continue;
+        const int ax = bci == 1 || bci == 3 ? 0 : bci == 0 || bci == 2 ? 1 : 2;
+        const int hi = bci == 0 || bci == 1 || bci == 5;
+        const int N = axis_n(s, ax);
+        const FacePos f = face_pos(N, hi);
+        const double bi = C1(axis_d(s, ax), f.q0) / 2;
+        int a0, a1, b0, b1;
+        face_ranges(s, ax, &a0, &a1, &b0, &b1);
+        const size_t ext = conv_extent(s, ax);
+        if (!s->conv_g[bci]) s->conv_g[bci] = calloc(3 * ext, sizeof(double));
+        double *g = s->conv_g[bci];
+        double uc = s->conv_uc[bci];
+        if (!init && uc <= 0) {
+            /* tangential widths of the face's interior cells */
+            const double *w1 = ax == 0 ? s->dy : s->dx, *w2 = ax == 2 ? s->dy : s->dz;
+            const int n1 = ax == 0 ? s->Ny : s->Nx, n2 = ax == 2 ? s->Ny : s->Nz;
+            double num = 0, den = 0;
+            for (int t1 = 0; t1 < n1; t1++)
+                for (int t2 = 0; t2 < n2; t2++) {
+                    const double A = C1(w1, t1) * C1(w2, t2);
+                    num += (hi ? 1 : -1) * g[(size_t)ax * ext + (size_t)(t1 - a0) * (b1 - b0) + (t2 - b0)] * A;
+                    den += A;
+                }
+            uc = fmax(num / den, 0.0);
+        }
+        for (int l = 0; l < 3; l++)
+            for (int t1 = a0; t1 < a1; t1++)
+                for (int t2 = b0; t2 < b1; t2++) {
+                    double *gv = &g[(size_t)l * ext + (size_t)(t1 - a0) * (b1 - b0) + (t2 - b0)];
+                    const double uq = u[l + 1][face_cell(s, ax, f.q0, t1, t2)];
+                    *gv = init ? uq : *gv - s->dt * uc * (*gv - uq) / bi;
+                }
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -520,7 +588,8 @@ static void mat_set_row(EllMat *A, int row, int n, const int *cols, const double
 
 static bool p_singular(const IBMSolver *s) {
     for (int i = 0; i < 6; i++)
-        if (s->bc[i].type == BC_PRESSURE || s->bc[i].type == BC_VELOCITY_PERIODIC) return false;
+        if (s->bc[i].type == BC_PRESSURE || s->bc[i].type == BC_VELOCITY_PERIODIC
+            || s->bc[i].type == BC_CONVECTIVE_OUTFLOW) return false;
     return true;  /* src/ibm3d_setup.c:1598-1603 */
 }
 
@@ -672,6 +741,9 @@ static void build_matrix(IBMSolver *s, int type) {
                         case BC_VELOCITY_PERIODIC:
                             if (type != 4) row_periodic(A, s, ax, hi, t1, t2, layer);
                             else row_dirichlet(A, s, ax, hi, t1, t2, layer);
+                            break;
+                        case BC_CONVECTIVE_OUTFLOW:
+                            row_dirichlet(A, s, ax, hi, t1, t2, layer);
                             break;
                         default:;
                         }
@@ -1017,7 +1089,8 @@ static void calc_u_star(IBMSolver *s) {
         for (int ax = 0; ax < 3; ax++)
             for (int hi = 0; hi < 2; hi++) {
                 const int bci = face_bc(ax, hi);
-                if (s->bc[bci].type != BC_VELOCITY_COMPONENT) continue;
+                const bool conv = s->bc[bci].type == BC_CONVECTIVE_OUTFLOW;
+                if (s->bc[bci].type != BC_VELOCITY_COMPONENT && !conv) continue;
                 const FacePos f = face_pos(axis_n(s, ax), hi);
                 int a0, a1, b0, b1;
                 face_ranges(s, ax, &a0, &a1, &b0, &b1);
@@ -1025,7 +1098,8 @@ static void calc_u_star(IBMSolver *s) {
                     for (int t2 = b0; t2 < b1; t2++) {
                         double x, y, z;
                         face_coord(s, ax, hi, t1, t2, &x, &y, &z);
-                        const double g = bc_val_u(s, l, bci, s->time, x, y, z);
+                        const double g = conv ? conv_val(s, bci, l, t1, t2)
+                                              : bc_val_u(s, l, bci, s->time, x, y, z);
                         s->b[face_cell(s, ax, f.o1, t1, t2)] = g;
                         s->b[face_cell(s, ax, f.o2, t1, t2)] = g;
                     }
@@ -1258,8 +1332,10 @@ static void outer_u_face(IBMSolver *s, int ax, int hi) {
                 for (int l = 1; l <= 3; l++) {
                     double *a = u[l];
                     switch (bt) {
-                    case BC_VELOCITY_COMPONENT: {
-                        const double g = bc_val_u(s, l, bci, s->time, x, y, z);
+                    case BC_VELOCITY_COMPONENT:
+                    case BC_CONVECTIVE_OUTFLOW: {
+                        const double g = bt == BC_CONVECTIVE_OUTFLOW ? conv_val(s, bci, l, t1, t2)
+                                                                     : bc_val_u(s, l, bci, s->time, x, y, z);
                         a[co] = (ao + bi) / bi * g - ao / bi * a[face_cell(s, ax, q, t1, t2)];
                         break;
                     }
@@ -1288,7 +1364,7 @@ static void outer_p_face(IBMSolver *s, int ax, int hi) {
     const int bci = face_bc(ax, hi);
     const IBMSolverBCType bt = s->bc[bci].type;
     const int N = axis_n(s, ax);
-    const double *d = (ax == 2 && !hi) ? s->dx : axis_d(s, ax);
+    const double *d = (ax == 2 && !hi && bt != BC_CONVECTIVE_OUTFLOW) ? s->dx : axis_d(s, ax);
     const FacePos f = face_pos(N, hi);
     double *p = s->p;
     int a0, a1, b0, b1;
@@ -1317,6 +1393,9 @@ static void outer_p_face(IBMSolver *s, int ax, int hi) {
                 }
                 case BC_ALL_PERIODIC:
                     p[co] = p[face_cell(s, ax, hi ? o - N : o + N, t1, t2)];
+                    break;
+                case BC_CONVECTIVE_OUTFLOW:  /* p = 0 on the face */
+                    p[co] = -ao / bi * p[face_cell(s, ax, q, t1, t2)];
                     break;
                 default:;
                 }
@@ -1375,6 +1454,8 @@ static void update_ghost(IBMSolver *s) {
 /* <- src/ibm3d_fracstep.c:38-115 */
 void IBMSolver_iterate(IBMSolver *s, int num_time_steps, bool verbose) {
     const double start_time = s->time;
+    if (!s->conv_ready) conv_outflow(s, 1);
+    s->conv_ready = true;
     update_outer(s);
     interp_stag_vel(s);
     update_ghost(s);
@@ -1382,6 +1463,7 @@ void IBMSolver_iterate(IBMSolver *s, int num_time_steps, bool verbose) {
     SWAPP(s->N1, s->N1_prev); SWAPP(s->N2, s->N2_prev); SWAPP(s->N3, s->N3_prev);
     for (int i = 0; i < num_time_steps;) {
         calc_N(s);
+        conv_outflow(s, 0);
         calc_u_star(s);
         calc_u_tilde(s);
         calc_U_star(s);
@@ -1411,11 +1493,13 @@ void IBMSolver_init_flow_const(IBMSolver *s, const double u1, const double u2,
                                const double u3, const double p) {
     s->iter = 0;
     s->time = 0;
+    s->conv_ready = false;
     for (size_t q = 0; q < ncell(s); q++) { s->u1[q] = u1; s->u2[q] = u2; s->u3[q] = u3; s->p[q] = p; }
 }
 
 void IBMSolver_init_flow_func(IBMSolver *s, IBMSolverInitFunc f1, IBMSolverInitFunc f2,
                               IBMSolverInitFunc f3, IBMSolverInitFunc fp) {
+    s->conv_ready = false;
     for (int i = -2; i < s->Nx + 2; i++)
         for (int j = -2; j < s->Ny + 2; j++)
             for (int k = -2; k < s->Nz + 2; k++) {
@@ -1476,6 +1560,7 @@ void IBMSolver_set_field(IBMSolver *s, int field, const double *in) {
                 : field == IBM3D_FIELD_U3 ? s->u3 : field == IBM3D_FIELD_P ? s->p : NULL;
     if (!dst) fatal("set_field supports u1, u2, u3, p");
     memcpy(dst, in, ncell(s) * sizeof(double));
+    s->conv_ready = false;
 }
 
 void IBMSolver_get_flag(IBMSolver *s, int *out) { memcpy(out, s->flag, ncell(s) * sizeof(int)); }

@@ -64,6 +64,11 @@ struct _ibm_solver {
     double *kx_W, *kx_E, *ky_S, *ky_N, *kz_D, *kz_U;
 
     OBC bc[6];
+    /* BC_CONVECTIVE_OUTFLOW faces (extension): convection speed (<= 0: mean
+       outward velocity) and face values, 3 components x face_ranges extent */
+    double conv_uc[6];
+    double *conv_g[6];
+    bool conv_ready;
 
     IBMSolverValType force_type;
     double const_f[3];
