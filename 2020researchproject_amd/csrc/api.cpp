@@ -211,10 +211,9 @@ void IBMSolver_set_mg_params(IBMSolver *s, double gamma, int pre, int post, int 
     s->mg_smoother = smoother;
 }
 
-void IBMSolver_set_mg_light_levels(IBMSolver *s, int from_level) { s->mg_light_from = from_level; }
-void IBMSolver_set_mixed_params(IBMSolver *s, double inner_tol, int momentum_fp32) {
-    s->mixed_inner_tol = inner_tol;
-    s->mixed_momentum_fp32 = momentum_fp32 != 0;
+void IBMSolver_set_mg_light_levels(IBMSolver *s, int from_level) {
+    if (s->assembled) fatal("IBMSolver_set_mg_light_levels must precede IBMSolver_assemble");
+    s->mg_light_from = from_level;
 }
 
 // Mirror point and trilinear weights  <- src/utils.c:111-171 (base indices

@@ -189,8 +189,6 @@ struct _ibm_solver {
     int mg_pre = 2, mg_post = 2; // smoothing sweeps per level
     int mg_smoother = 1;         // 0 fused red-black sweeps, 1 one launch per colour
     int mg_light_from = 0;       // levels >= this (if > 0) smooth with one sweep before and after
-    double mixed_inner_tol = 1e-4;  // IBM3D_PREC_MIXED: relative tolerance of the FP32 inner solves
-    bool mixed_momentum_fp32 = true;  // IBM3D_PREC_MIXED: momentum solves refined in FP32 too
 
     // ---- host mirrors ----
     std::vector<double> lvset_h;     // reference layout (k fastest)

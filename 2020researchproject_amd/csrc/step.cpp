@@ -745,7 +745,7 @@ static int refine_mixed(IBMSolver *s, int sk, const double *b, double *x, int pr
     double rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
     int total = 0, outer = 0;
     const NormSpec off{0, nullptr, nullptr, nullptr, nullptr, -1};
-    const double inner_tol = s->mixed_inner_tol;
+    const double inner_tol = 1e-4;
     while (rn > tol * bn && outer < 20 && total < maxit) {
         outer++;
         TIMED("d2f", cb * 12, launch_d2f(s->stream, d, r64, rhs32));
@@ -776,7 +776,7 @@ static void solve_system(IBMSolver *s, int sk, const double *b, double *x, doubl
     const int maxit = sk == 3 ? 1000 : 200;  // src/ibm3d_setup.c:1413, 1467
     double rel = 0;
     int it;
-    if (s->precision == IBM3D_PREC_MIXED && (sk == 3 || s->mixed_momentum_fp32)) {
+    if (s->precision == IBM3D_PREC_MIXED) {
         it = refine_mixed(s, sk, b, x, prec, ns, s->tol, maxit, bnorm, &rel);
     } else if (s->precision == IBM3D_PREC_MIXED_KRYLOV) {
         const GhostData &GD = sk == 3 ? s->gp : s->gmom;

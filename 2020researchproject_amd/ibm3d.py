@@ -90,6 +90,7 @@ _OPTIONAL = {
                                             C.POINTER(C.c_long), _dp]),
     "IBMSolver_get_device_bytes": (C.c_double, [_vp]),
     "IBMSolver_get_mg_levels": (C.c_int, [_vp, _ip, C.c_int]),
+    "IBMSolver_set_mg_light_levels": (None, [_vp, C.c_int]),
     "IBMSolver_estimate_device_bytes": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "IBMServer_new": (_vp, [C.c_double, C.c_int]),
     "IBMServer_submit": (C.c_int, [_vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, _vp]),
@@ -226,6 +227,10 @@ class Solver:
 
     def set_mg_params(self, gamma: float = 3.0, pre: int = 2, post: int = 2, smoother: int = 1):
         self.lib.IBMSolver_set_mg_params(self.h, gamma, pre, post, smoother)
+
+    def set_mg_light_levels(self, from_level: int = 0):
+        if hasattr(self.lib, "IBMSolver_set_mg_light_levels"):
+            self.lib.IBMSolver_set_mg_light_levels(self.h, int(from_level))
 
     def assemble(self):
         self.lib.IBMSolver_assemble(self.h)
