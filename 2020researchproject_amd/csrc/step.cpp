@@ -195,6 +195,9 @@ static void read_ks(IBMSolver *s) {
 // coarsening stops at the first level below the fine grid with at most this
 // many cells; that level is solved exactly through a dense inverse
 static const long kDenseCells = 3000;
+// levels with at least this many cells restrict their residual with two
+// launches (residual, then restriction), smaller ones with one
+static const long kSplitRestrictCells = 1L << 20;
 
 static void level_geometry(IBMSolver *s, MGLevel &L, const std::vector<double> &xf, const std::vector<double> &yf,
                            const std::vector<double> &zf) {
@@ -420,7 +423,7 @@ void build_mg(IBMSolver *s) {
                         far[i] = (J + m[a]) % m[a];
                     } else {
                         const IBMSolverBCType t = s->bc[J < 0 ? lo[a] : hi[a]].type;
-                        const bool dir = t == BC_PRESSURE || t == BC_VELOCITY_PERIODIC;
+                        const bool dir = t == BC_PRESSURE || t == BC_VELOCITY_PERIODIC || t == BC_CONVECTIVE_OUTFLOW;
                         w[i] = dir ? wn - (1 - wn) : 1.0;  // Dirichlet: odd mirror, Neumann: even
                         far[i] = I;
                     }
@@ -559,12 +562,15 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
     const bool light = s->mg_light_from > 0 && l >= s->mg_light_from;
     smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, light ? 1 : s->mg_pre, true, l);
     MGLevel &Cl = s->mg[l + 1];
-    if (fine) {
-        // fine level: residual, then restriction (measured faster than the
+    if ((long)L.d.nx * L.d.ny * L.d.nz >= kSplitRestrictCells) {
+        // large levels: residual, then restriction (measured faster than the
         // one-pass kernel, whose per-coarse-cell child loops run short here)
-        TIMED("mg_resid_fine", 3 * cell_b + cell_b / sizeof(T),
+        char rn[32], tn[32];
+        snprintf(rn, sizeof rn, "mg_resid_L%d", l);
+        snprintf(tn, sizeof tn, "mg_restrict_L%d", l);
+        TIMED(rn, 3 * cell_b + cell_b / sizeof(T),
               launch_mg_residual<T>(s->stream, L.d, S, flag, b, x, lev_r<T>(L).p));
-        TIMED("mg_restrict", cell_b + cell_b / sizeof(T),
+        TIMED(tn, cell_b + cell_b / sizeof(T),
               launch_restrict<T>(s->stream, L.d, Cl.d, Cl.tr, flag, lev_r<T>(L).p, L.dxyz.p, L.dxyz.p + (L.d.nx + 4),
                                  L.dxyz.p + (L.d.nx + 4) + (L.d.ny + 4), Cl.dxyz.p, Cl.dxyz.p + (Cl.d.nx + 4),
                                  Cl.dxyz.p + (Cl.d.nx + 4) + (Cl.d.ny + 4), lev_b<T>(Cl).p));
