@@ -418,21 +418,31 @@ static void build_ghosts(IBMSolver *s) {
 
     // solver ghost rows (same stencils for every system)
     std::vector<long> cell(ng), src(8 * (size_t)ng);
+    std::vector<int> srcg(8 * (size_t)ng, -1);
     std::vector<double> w64(8 * (size_t)ng), ws64(ng);
+    std::unordered_map<long, int> slot;  // ghost cell -> ghost row
+    for (int q = 0; q < ng; q++) slot[s->ghosts[q].cell] = q;
     for (int q = 0; q < ng; q++) {
         cell[q] = s->ghosts[q].cell;
         ws64[q] = s->ghosts[q].wself;
-        for (int l = 0; l < 8; l++) { src[8 * q + l] = s->ghosts[q].src[l]; w64[8 * q + l] = s->ghosts[q].w[l]; }
+        for (int l = 0; l < 8; l++) {
+            src[8 * q + l] = s->ghosts[q].src[l];
+            w64[8 * q + l] = s->ghosts[q].w[l];
+            const auto it = slot.find(s->ghosts[q].src[l]);
+            if (s->ghosts[q].src[l] >= 0 && it != slot.end()) srcg[8 * q + l] = it->second;
+        }
     }
     std::vector<float> w32(w64.begin(), w64.end()), ws32(ws64.begin(), ws64.end());
     for (GhostData *G : {&s->gmom, &s->gp}) {
-        G->cell.upload(cell); G->src.upload(src);
+        G->cell.upload(cell); G->src.upload(src); G->srcg.upload(srcg);
         G->w64.upload(w64); G->ws64.upload(ws64);
         G->w32.upload(w32); G->ws32.upload(ws32);
-        G->t64.alloc(std::max(ng, 1));
-        G->t32.alloc(std::max(ng, 1));
-        G->G64 = {ng, G->cell.p, G->src.p, G->w64.p, G->ws64.p, G->t64.p};
-        G->G32 = {ng, G->cell.p, G->src.p, G->w32.p, G->ws32.p, G->t32.p};
+        for (int k = 0; k < 2; k++) {
+            G->v64[k].alloc(std::max(ng, 1));
+            G->v32[k].alloc(std::max(ng, 1));
+        }
+        G->G64 = {ng, G->cell.p, G->src.p, G->w64.p, G->ws64.p, G->srcg.p, {G->v64[0].p, G->v64[1].p}};
+        G->G32 = {ng, G->cell.p, G->src.p, G->w32.p, G->ws32.p, G->srcg.p, {G->v32[0].p, G->v32[1].p}};
     }
 }
 

@@ -105,8 +105,9 @@ struct SysData {
 
 struct GhostData {
     DBuf<long> cell, src;
-    DBuf<double> w64, ws64, t64;
-    DBuf<float> w32, ws32, t32;
+    DBuf<int> srcg;
+    DBuf<double> w64, ws64, v64[2];
+    DBuf<float> w32, ws32, v32[2];
     GhostRows<double> G64{};
     GhostRows<float> G32{};
 };
@@ -223,12 +224,11 @@ struct _ibm_solver {
     ibm::DBuf<ibm::GhostCell> gc;
     ibm::DBuf<double> ghost_old;
     ibm::GhostData gmom, gp;              // solver ghost rows (momentum / pressure)
+    int ghost_prev = -1;                  // where the pressure ghost values are (launch_ghost_relax)
 
     ibm::SysData sys[4];                  // u1, u2, u3, p
     long pin = -1;
     std::vector<ibm::MGLevel> mg;         // mg[0] = fine pressure level
-    ibm::DBuf<ibm::LevelDev<double>> mg_lv64;
-    ibm::DBuf<ibm::LevelDev<float>> mg_lv32;
     std::vector<ibm::VGraph> vgraphs;     // captured V-cycles per (input, output)
     ibm::Krylov kry;
     ibm::DBuf<double> ref_r64;            // mixed mode: FP64 defect

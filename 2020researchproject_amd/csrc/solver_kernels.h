@@ -14,7 +14,8 @@ struct GhostRows {
     const long *src;    // 8 * ng (-1 = unused)
     const T *w;         // 8 * ng
     const T *wself;     // ng
-    T *tmp;             // ng: new ghost values of one relaxation, scattered after the gather
+    const int *srcg;    // 8 * ng: ghost row of the source cell, -1 for a fluid source
+    T *vals[2];         // ng each: ghost values of the last two relaxations
 };
 
 // Device-resident scalars of BiCGSTAB.
@@ -58,12 +59,12 @@ void launch_rbgs_color(hipStream_t st, const Dims &d, const SysCoef<T> &S, const
 template <typename T>
 void launch_rbgs_fused(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
                        const T *xa, T *xb, int zero_in);
-// Jacobi relaxation of the ghost rows (gather into G.tmp, then scatter)
+// Jacobi relaxation of the ghost rows in one launch. `prev` names where the
+// current ghost values are: -1 all zero, 0 or 1 G.vals[prev] (they equal x
+// at the ghost cells). The new values go to x and to the other buffer, whose
+// index is returned: the next relaxation's `prev`.
 template <typename T>
-void launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x);
-template <typename T>
-void launch_rbgs(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
-                 const T *b, T *x, int sweeps);
+int launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x, int prev);
 
 // BiCGSTAB pieces
 template <typename T>
@@ -136,24 +137,5 @@ void launch_d2f_flat(hipStream_t st, const double *a, float *b, long n);
 template <typename T>
 void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, const double *vol,
                         const uint8_t *flag, int singular, int pinrow, const T *b, T *x);
-
-// One multigrid level as the single-workgroup tail V-cycle sees it. `tr`
-// is the transfer from the next finer level to this one.
-template <typename T>
-struct LevelDev {
-    Dims d;
-    SysCoef<T> S;
-    const uint8_t *flag;
-    T *x, *b, *r;
-    const double *dx, *dy, *dz;  // padded 1-D widths (index i + 2)
-    MGTransfer tr;
-};
-
-// V-cycle over levels l0 .. nl-1 of `lv` in one workgroup (levels small
-// enough that launch latency, not bandwidth, would dominate). b of level l0
-// must be set; writes x of level l0.
-template <typename T>
-void launch_tail_vcycle(hipStream_t st, const LevelDev<T> *lv, int l0, int nl, int pre, int post, int coarse_sweeps,
-                        int singular);
 
 }  // namespace ibm

@@ -404,18 +404,22 @@ void launch_rbgs_fused(hipStream_t st, const Dims &d, const SysCoef<T> &S, const
 }
 
 // Jacobi relaxation of the ghost rows. A ghost stencil may read other ghost
-// cells, so the new values are gathered into G.tmp first and scattered by a
-// second launch; updating x in place would make the result depend on the
+// cells, which this launch also writes: those sources are read from the
+// values the previous relaxation left in a separate buffer (no other kernel
+// writes x at ghost cells in between), so the result does not depend on the
 // order in which waves run.
 template <typename T>
-__global__ void k_ghost_relax(GhostRows<T> G, int is_p, const T *b, const T *x) {
+__global__ void k_ghost_relax(GhostRows<T> G, int is_p, const T *b, T *x, const T *prev, T *next) {
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= G.ng) return;
     T a = 0;
 #pragma unroll
     for (int l = 0; l < 8; l++) {
         const long c = G.src[8 * (long)s + l];
-        if (c >= 0) a += G.w[8 * (long)s + l] * x[c];
+        if (c < 0) continue;
+        const int g = G.srcg[8 * (long)s + l];
+        const T v = g < 0 ? x[c] : prev ? prev[g] : T(0);
+        a += G.w[8 * (long)s + l] * v;
     }
     const long c = G.cell[s];
     const T ws = G.wself[s];
@@ -426,14 +430,8 @@ __global__ void k_ghost_relax(GhostRows<T> G, int is_p, const T *b, const T *x) 
     } else {
         v = (b[c] - a) / (T(1) + ws);
     }
-    G.tmp[s] = v;
-}
-
-template <typename T>
-__global__ void k_ghost_scatter(GhostRows<T> G, T *x) {
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= G.ng) return;
-    x[G.cell[s]] = G.tmp[s];
+    x[c] = v;
+    next[s] = v;
 }
 
 template <typename T>
@@ -447,21 +445,13 @@ void launch_rbgs_color(hipStream_t st, const Dims &d, const SysCoef<T> &S, const
 }
 
 template <typename T>
-void launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x) {
-    if (G.ng <= 0) return;
+int launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x, int prev) {
+    const int out = prev == 0 ? 1 : 0;
+    if (G.ng <= 0) return out;
     const int gb = (G.ng + 255) / 256;
-    hipLaunchKernelGGL((k_ghost_relax<T>), dim3(gb), dim3(256), 0, st, G, is_p, b, x);
-    hipLaunchKernelGGL((k_ghost_scatter<T>), dim3(gb), dim3(256), 0, st, G, x);
-}
-
-template <typename T>
-void launch_rbgs(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
-                 const T *b, T *x, int sweeps) {
-    for (int s = 0; s < sweeps; s++)
-        for (int c = 0; c < 2; c++) {
-            launch_rbgs_color<T>(st, d, S, flag, b, x, c, 0);
-            launch_ghost_relax<T>(st, G, S.is_p, b, x);
-        }
+    hipLaunchKernelGGL((k_ghost_relax<T>), dim3(gb), dim3(256), 0, st, G, is_p, b, x,
+                       prev < 0 ? (const T *)nullptr : (const T *)G.vals[prev], G.vals[out]);
+    return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -855,219 +845,6 @@ void launch_coarse_solve(hipStream_t st, const Dims &d, const SysCoef<T> &S, con
 }
 
 // ---------------------------------------------------------------------------
-// tail V-cycle: every level from l0 down to the coarsest and back up in one
-// workgroup; phases separated by workgroup barriers (global memory stays
-// coherent within one CU).
-// ---------------------------------------------------------------------------
-constexpr int kLdsCoarse = 512;  // coarsest grids up to this size are swept in LDS
-
-__device__ inline void decode(long c, const Dims &d, int &i, int &j, int &k) {
-    i = c % d.nx;
-    j = (c / d.nx) % d.ny;
-    k = c / ((long)d.nx * d.ny);
-}
-
-template <typename T>
-__device__ void tv_smooth(const LevelDev<T> &L, int sweeps, T bshift) {
-    const Dims &d = L.d;
-    const long n = (long)d.nx * d.ny * d.nz;
-    for (int s = 0; s < sweeps; s++)
-        for (int color = 0; color < 2; color++) {
-            for (long c = threadIdx.x; c < n; c += blockDim.x) {
-                int i, j, k;
-                decode(c, d, i, j, k);
-                if (((i + j + k) & 1) != color) continue;
-                const long q = cidx(d, i, j, k);
-                if (q == L.S.pin || L.flag[q] == F_SOLID) { L.x[q] = L.b[q]; continue; }
-                T diag, off;
-                stencil(d, L.S, L.flag, i, j, k, q, L.x, diag, off);
-                L.x[q] = diag != T(0) ? (L.b[q] - bshift + off) / diag : T(0);
-            }
-            __syncthreads();
-        }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(1024) k_tail_vcycle(const LevelDev<T> *lv, int l0, int nl, int pre, int post,
-                                                      int coarse_sweeps, int singular) {
-    __shared__ double red[2][1024];
-    const int t = threadIdx.x;
-    // downward leg
-    for (int l = l0; l < nl - 1; l++) {
-        const LevelDev<T> &L = lv[l];
-        const Dims &d = L.d;
-        const long n = (long)d.nx * d.ny * d.nz;
-        for (long c = t; c < n; c += blockDim.x) {
-            int i, j, k;
-            decode(c, d, i, j, k);
-            L.x[cidx(d, i, j, k)] = 0;
-        }
-        __syncthreads();
-        tv_smooth(L, pre, T(0));
-        for (long c = t; c < n; c += blockDim.x) {
-            int i, j, k;
-            decode(c, d, i, j, k);
-            const long q = cidx(d, i, j, k);
-            T rv = 0;
-            if (q == L.S.pin || L.flag[q] == F_SOLID) rv = L.b[q] - L.x[q];
-            else {
-                T diag, off;
-                stencil(d, L.S, L.flag, i, j, k, q, L.x, diag, off);
-                rv = L.b[q] - (diag * L.x[q] - off);
-            }
-            L.r[q] = rv;
-        }
-        __syncthreads();
-        const LevelDev<T> &Cl = lv[l + 1];
-        const Dims &dc = Cl.d;
-        const long nc = (long)dc.nx * dc.ny * dc.nz;
-        for (long c = t; c < nc; c += blockDim.x) {
-            int I, J, K;
-            decode(c, dc, I, J, K);
-            double acc = 0;
-            const MGTransfer &tr = Cl.tr;
-            for (int k = tr.sk[K]; k < tr.sk[K + 1]; k++)
-                for (int j = tr.sj[J]; j < tr.sj[J + 1]; j++)
-                    for (int i = tr.si[I]; i < tr.si[I + 1]; i++) {
-                        const long q = cidx(d, i, j, k);
-                        if (L.flag[q] != F_FLUID) continue;
-                        acc += (double)L.r[q] * L.dx[i + 2] * L.dy[j + 2] * L.dz[k + 2];
-                    }
-            Cl.b[cidx(dc, I, J, K)] = (T)(acc / (Cl.dx[I + 2] * Cl.dy[J + 2] * Cl.dz[K + 2]));
-        }
-        __syncthreads();
-    }
-    // coarsest level
-    {
-        const LevelDev<T> &L = lv[nl - 1];
-        const Dims &d = L.d;
-        const long n = (long)d.nx * d.ny * d.nz;
-        double bmean = 0;
-        if (singular) {
-            double sv = 0, sb = 0;
-            for (long c = t; c < n; c += blockDim.x) {
-                int i, j, k;
-                decode(c, d, i, j, k);
-                const long q = cidx(d, i, j, k);
-                if (L.flag[q] != F_FLUID) continue;
-                const double v = L.dx[i + 2] * L.dy[j + 2] * L.dz[k + 2];
-                sv += v; sb += v * (double)L.b[q];
-            }
-            red[0][t] = sv; red[1][t] = sb;
-            __syncthreads();
-            for (int s = 512; s > 0; s >>= 1) {
-                if (t < s) { red[0][t] += red[0][t + s]; red[1][t] += red[1][t + s]; }
-                __syncthreads();
-            }
-            bmean = red[0][0] > 0 ? red[1][0] / red[0][0] : 0;
-            __syncthreads();
-        }
-        if (n <= kLdsCoarse) {
-            // coarsest grid in LDS: rows gathered once, sweeps without global traffic
-            __shared__ double cx_[kLdsCoarse], cb_[kLdsCoarse], cd_[kLdsCoarse], cc_[6][kLdsCoarse];
-            __shared__ int cn_[6][kLdsCoarse];
-            for (long c = t; c < n; c += blockDim.x) {
-                int i, j, k;
-                decode(c, d, i, j, k);
-                const long q = cidx(d, i, j, k);
-                cx_[c] = 0;
-                const bool ident = q == L.S.pin || L.flag[q] == F_SOLID;
-                cb_[c] = ident ? (double)L.b[q] : (double)L.b[q] - bmean;
-                const int id[3] = {i, j, k}, nn[3] = {d.nx, d.ny, d.nz};
-                double diag = ident ? 1.0 : (double)L.S.c0;
-                for (int a = 0; a < 3; a++) {
-                    const AxisCoef<T> &A = L.S.ax[a];
-                    const int ii = id[a], n_ = nn[a];
-                    int im = ii - 1, ip = ii + 1;
-                    bool hm = ii > 0, hp = ii < n_ - 1;
-                    if (A.periodic) { im = (ii + n_ - 1) % n_; ip = (ii + 1) % n_; hm = hp = true; }
-                    double cm = A.cm[ii], cp = A.cp[ii], dm = A.dm[ii], dp = A.dp[ii];
-                    int jm[3] = {i, j, k}, jp[3] = {i, j, k};
-                    jm[a] = im; jp[a] = ip;
-                    const long cm_idx = ((long)jm[2] * d.ny + jm[1]) * d.nx + jm[0];
-                    const long cp_idx = ((long)jp[2] * d.ny + jp[1]) * d.nx + jp[0];
-                    if (hm && L.S.skip && L.flag[cidx(d, jm[0], jm[1], jm[2])] == F_SOLID) { cm = 0; dm = 0; }
-                    if (hp && L.S.skip && L.flag[cidx(d, jp[0], jp[1], jp[2])] == F_SOLID) { cp = 0; dp = 0; }
-                    if (ident) { cm = cp = 0; }
-                    else diag += dm + dp;
-                    cn_[2 * a][c] = hm ? (int)cm_idx : -1;
-                    cn_[2 * a + 1][c] = hp ? (int)cp_idx : -1;
-                    cc_[2 * a][c] = hm ? cm : 0;
-                    cc_[2 * a + 1][c] = hp ? cp : 0;
-                }
-                cd_[c] = diag;
-            }
-            __syncthreads();
-            for (int s = 0; s < coarse_sweeps; s++)
-                for (int color = 0; color < 2; color++) {
-                    for (long c = t; c < n; c += blockDim.x) {
-                        int i, j, k;
-                        decode(c, d, i, j, k);
-                        if (((i + j + k) & 1) != color) continue;
-                        double acc = cb_[c];
-                        for (int m = 0; m < 6; m++)
-                            if (cn_[m][c] >= 0) acc += cc_[m][c] * cx_[cn_[m][c]];
-                        cx_[c] = cd_[c] != 0 ? acc / cd_[c] : 0;
-                    }
-                    __syncthreads();
-                }
-            for (long c = t; c < n; c += blockDim.x) {
-                int i, j, k;
-                decode(c, d, i, j, k);
-                L.x[cidx(d, i, j, k)] = (T)cx_[c];
-            }
-            __syncthreads();
-        } else {
-            for (long c = t; c < n; c += blockDim.x) {
-                int i, j, k;
-                decode(c, d, i, j, k);
-                L.x[cidx(d, i, j, k)] = 0;
-            }
-            __syncthreads();
-            tv_smooth(L, coarse_sweeps, (T)bmean);
-        }
-    }
-    // upward leg
-    for (int l = nl - 2; l >= l0; l--) {
-        const LevelDev<T> &L = lv[l];
-        const LevelDev<T> &Cl = lv[l + 1];
-        const Dims &d = L.d, &dc = Cl.d;
-        const long n = (long)d.nx * d.ny * d.nz;
-        const MGTransfer &tr = Cl.tr;
-        for (long c = t; c < n; c += blockDim.x) {
-            int i, j, k;
-            decode(c, d, i, j, k);
-            const long q = cidx(d, i, j, k);
-            if (L.flag[q] != F_FLUID) continue;
-            const int I = tr.ci[i], J = tr.cj[j], K = tr.ck[k];
-            const int Is[2] = {I, tr.pi[i]}, Js[2] = {J, tr.pj[j]}, Ks[2] = {K, tr.pk[k]};
-            const double wi[2] = {tr.wi[i], 1 - tr.wi[i]}, wj[2] = {tr.wj[j], 1 - tr.wj[j]},
-                         wk[2] = {tr.wk[k], 1 - tr.wk[k]};
-            const double e0 = (double)Cl.x[cidx(dc, I, J, K)];
-            double e = 0;
-            for (int cz = 0; cz < 2; cz++)
-                for (int cy = 0; cy < 2; cy++)
-                    for (int cx = 0; cx < 2; cx++) {
-                        const double w = wi[cx] * wj[cy] * wk[cz];
-                        if (w == 0) continue;
-                        const long qc = cidx(dc, Is[cx], Js[cy], Ks[cz]);
-                        e += w * (Cl.flag[qc] == F_SOLID ? e0 : (double)Cl.x[qc]);
-                    }
-            L.x[q] += (T)e;
-        }
-        __syncthreads();
-        tv_smooth(L, post, T(0));
-    }
-}
-
-template <typename T>
-void launch_tail_vcycle(hipStream_t st, const LevelDev<T> *lv, int l0, int nl, int pre, int post, int coarse_sweeps,
-                        int singular) {
-    hipLaunchKernelGGL((k_tail_vcycle<T>), dim3(1), dim3(1024), 0, st, lv, l0, nl, pre, post, coarse_sweeps,
-                       singular);
-}
-
-// ---------------------------------------------------------------------------
 // exact coarsest-level solve through a precomputed dense inverse
 // ---------------------------------------------------------------------------
 __global__ void k_gj_prep(const double *A, int n, int p, double *rowp, double *colp) {
@@ -1168,7 +945,6 @@ void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, 
 
 // explicit instantiations
 #define INST(T)                                                                                                   \
-    template void launch_tail_vcycle<T>(hipStream_t, const LevelDev<T> *, int, int, int, int, int, int);            \
     template void launch_dense_solve<T>(hipStream_t, const T *, int, const long *, const double *, const uint8_t *, \
                                         int, int, const T *, T *);                                                \
     template void launch_rbgs_fused<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *, const T *,  \
@@ -1180,15 +956,13 @@ void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, 
     template void launch_resid_restrict<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *,         \
                                            const T *, const T *, const Dims &, const MGTransfer &, const double *, \
                                            const double *, T *);                                                  \
-    template void launch_ghost_relax<T>(hipStream_t, const GhostRows<T> &, int, const T *, T *);                   \
+    template int launch_ghost_relax<T>(hipStream_t, const GhostRows<T> &, int, const T *, T *, int);               \
     template void launch_apply<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,             \
                                   const uint8_t *, const T *, T *, int, const T *, double *);                      \
     template void launch_residual<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,          \
                                      const uint8_t *, const T *, const T *, T *, const NormSpec &, double *);       \
     template void launch_jacobi<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,            \
                                    const uint8_t *, const T *, T *);                                               \
-    template void launch_rbgs<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,              \
-                                 const uint8_t *, const T *, T *, int);                                            \
     template void launch_bicg_restart<T>(hipStream_t, const Dims &, const T *, T *, T *, T *);                \
     template void launch_bicg_init<T>(hipStream_t, const Dims &, T *, T *, T *, T *, T *, const T *);              \
     template void launch_bicg_p<T>(hipStream_t, const Dims &, const KScal *, const T *, const T *, T *);           \

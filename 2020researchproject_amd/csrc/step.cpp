@@ -455,18 +455,6 @@ void build_mg(IBMSolver *s) {
         s->mg[0].r32.alloc(s->d.ncell);
         s->mg[0].xa32.alloc(s->d.ncell);
     }
-    // level descriptors of the single-workgroup tail V-cycle (levels >= 1)
-    const int nl = (int)s->mg.size();
-    std::vector<LevelDev<double>> l64(nl);
-    std::vector<LevelDev<float>> l32(nl);
-    for (int l = 1; l < nl; l++) {
-        MGLevel &L = s->mg[l];
-        const double *dx = L.dxyz.p, *dy = dx + L.d.nx + 4, *dz = dy + L.d.ny + 4;
-        l64[l] = {L.d, L.sys.S64, L.flag.p, L.x64.p, L.b64.p, L.r64.p, dx, dy, dz, L.tr};
-        l32[l] = {L.d, L.sys.S32, L.flag.p, L.x32.p, L.b32.p, L.r32.p, dx, dy, dz, L.tr};
-    }
-    s->mg_lv64.upload(l64);
-    s->mg_lv32.upload(l32);
 }
 
 template <typename T> static const T *dinv(const MGLevel &L);
@@ -497,9 +485,6 @@ template <> DBuf<float> *kvec<float>(Krylov &K) { return K.v32; }
 static const int kCoarse = 40;
 
 // x = V-cycle approximation of A^-1 b on level l (x0 = 0)
-template <typename T> static const LevelDev<T> *lvdev(IBMSolver *s);
-template <> const LevelDev<double> *lvdev<double>(IBMSolver *s) { return s->mg_lv64.p; }
-template <> const LevelDev<float> *lvdev<float>(IBMSolver *s) { return s->mg_lv32.p; }
 
 // Red-black Gauss-Seidel sweeps on x (zero_init: the initial x is zero and
 // is not read). Smoother 0: one fused out-of-place launch per sweep,
@@ -512,8 +497,12 @@ static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const Ghost
                    const T *b, T *x, T *xalt, int sweeps, bool zero_init, int level) {
     const double n = (double)d.nx * d.ny * d.nz;
     char nm[32];
+    // a zero start also zeroes the ghost values (s->ghost_prev = -1)
     if (sweeps == 0) {
-        if (zero_init) TIMED("mg_zero", n * sizeof(T), launch_zero<T>(s->stream, d, x));
+        if (zero_init) {
+            TIMED("mg_zero", n * sizeof(T), launch_zero<T>(s->stream, d, x));
+            if (G.ng) s->ghost_prev = -1;
+        }
         return;
     }
     if (s->mg_smoother == 1) {
@@ -524,7 +513,9 @@ static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const Ghost
                 snprintf(nm, sizeof nm, "rbgs_L%d", level);
                 TIMED(nm, z ? n * (1.5 * sizeof(T) + 1) : n * (2 * sizeof(T) + 1),
                       launch_rbgs_color<T>(s->stream, d, S, flag, b, x, c, z ? 1 : 0));
-                if (G.ng) TIMED("ghost_relax", 0, launch_ghost_relax<T>(s->stream, G, S.is_p, b, x));
+                if (!G.ng) continue;
+                if (z) s->ghost_prev = -1;
+                TIMED("ghost_relax", 0, s->ghost_prev = launch_ghost_relax<T>(s->stream, G, S.is_p, b, x, s->ghost_prev));
             }
         return;
     }
@@ -534,7 +525,10 @@ static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const Ghost
         snprintf(nm, sizeof nm, "rbgs_fused_L%d", level);
         TIMED(nm, n * ((z ? 2 : 3) * sizeof(T) + 1),
               launch_rbgs_fused<T>(s->stream, d, S, flag, b, cur, nxt, z ? 1 : 0));
-        if (G.ng) TIMED("ghost_relax", 0, launch_ghost_relax<T>(s->stream, G, S.is_p, b, nxt));
+        if (G.ng) {
+            if (z) s->ghost_prev = -1;
+            TIMED("ghost_relax", 0, s->ghost_prev = launch_ghost_relax<T>(s->stream, G, S.is_p, b, nxt, s->ghost_prev));
+        }
         std::swap(cur, nxt);
     }
     if (cur != x) TIMED("mg_copy", 2 * n * sizeof(T), launch_copy<T>(s->stream, d, cur, x));
@@ -545,7 +539,7 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
     MGLevel &L = s->mg[l];
     const bool fine = l == 0;
     const SysCoef<T> &S = fine ? sysc<T>(s->sys[3]) : sysc<T>(L.sys);
-    const GhostRows<T> G0 = fine ? grows<T>(s->gp) : GhostRows<T>{0, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const GhostRows<T> G0 = fine ? grows<T>(s->gp) : GhostRows<T>{};
     const uint8_t *flag = fine ? s->flag.p : L.flag.p;
     const double cell_b = (double)L.d.nx * L.d.ny * L.d.nz * sizeof(T);
     if (l >= 1 && L.dn > 0) {
