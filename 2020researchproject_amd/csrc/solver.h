@@ -186,7 +186,7 @@ struct _ibm_solver {
     size_t dev_bytes = 0;        // device memory allocated during setup
     double vd_aplus = 26, vd_utau = 1;  // van Driest damping (TURBMODEL_SMAGORINSKY_VD)
     int vd_walls = 0;
-    double mg_gamma = 3.0;       // coarsening threshold (multiples of the level's smallest width)
+    double mg_gamma = 3.5;       // coarsening threshold (multiples of the level's smallest width)
     int mg_pre = 2, mg_post = 2; // smoothing sweeps per level
     int mg_smoother = 1;         // 0 fused red-black sweeps, 1 one launch per colour
     int mg_light_from = 0;       // levels >= this (if > 0) smooth with one sweep before and after

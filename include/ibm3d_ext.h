@@ -54,7 +54,7 @@ void IBMSolver_set_precision(IBMSolver *, int mode);
    red-black Gauss-Seidel sweeps before / after the coarse correction;
    smoother 0: each sweep is one fused launch (both colours), ghost rows
    relaxed after each sweep; 1: one launch per colour, ghost rows relaxed
-   after each colour. Defaults 3.0, 2, 2, 1. Must be called before
+   after each colour. Defaults 3.5, 2, 2, 1. Must be called before
    IBMSolver_assemble. */
 void IBMSolver_set_mg_params(IBMSolver *, double gamma, int pre, int post, int smoother);
 /* Levels from `from_level` down (when from_level > 0) smooth with one sweep
