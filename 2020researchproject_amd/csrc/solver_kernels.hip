@@ -19,6 +19,23 @@ namespace ibm {
 
 #define C1(a, i) ((a)[(i) + 2])
 
+// Workgroups of a launch go to the 8 XCDs round-robin by linear workgroup id
+// (x fastest), and each XCD has its own L2. xcd_block() maps the hardware id
+// to a logical block so that each XCD runs one contiguous range of logical
+// blocks (x fastest, then y, then z): the planes k-1, k, k+1 a stencil sweep
+// reads are then fetched by one XCD and reused from its L2, whatever the
+// number of blocks per plane.
+constexpr unsigned kNumXcd = 8;
+__device__ inline uint3 xcd_block(int on) {
+    if (!on) return make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned n = gx * gy * gridDim.z;
+    const unsigned w = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned x = w % kNumXcd, s = w / kNumXcd, q = n / kNumXcd, r = n % kNumXcd;
+    const unsigned L = x * q + (x < r ? x : r) + s;
+    return make_uint3(L % gx, (L / gx) % gy, L / (gx * gy));
+}
+
 __device__ inline long blin() {
     return blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
 }

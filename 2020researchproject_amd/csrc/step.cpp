@@ -330,18 +330,30 @@ void build_mg(IBMSolver *s) {
         for (int a = 0; a < 3; a++) {
             const std::vector<double> &f = *F[a];
             const bool full = s->mg_gamma <= 0 && n[a] % 2 == 0 && n[a] >= 2;
-            std::vector<double> g{f[0]};
-            parent[a].assign(n[a], 0);
-            for (int i = 0; i < n[a];) {
-                const bool pair = i + 1 < n[a] && (full || (s->mg_gamma > 0 && f[i + 2] - f[i] <= thr));
-                start[a].push_back(i);
-                parent[a][i] = (int)start[a].size() - 1;
-                if (pair) { parent[a][i + 1] = parent[a][i]; merged = true; }
-                i += pair ? 2 : 1;
-                g.push_back(f[i]);
+            // a periodic axis keeps an even cell count, so the two neighbours
+            // across the wrap have different red-black colours
+            const bool per = s->bc[lo[a]].type == BC_ALL_PERIODIC && s->bc[hi[a]].type == BC_ALL_PERIODIC;
+            for (int pass = 0; pass < 2; pass++) {
+                const bool allow = pass == 0;
+                std::vector<double> g{f[0]};
+                parent[a].assign(n[a], 0);
+                start[a].clear();
+                bool any = false;
+                for (int i = 0; i < n[a];) {
+                    const bool pair = allow && i + 1 < n[a] &&
+                                      (full || (s->mg_gamma > 0 && f[i + 2] - f[i] <= thr));
+                    start[a].push_back(i);
+                    parent[a][i] = (int)start[a].size() - 1;
+                    if (pair) { parent[a][i + 1] = parent[a][i]; any = true; }
+                    i += pair ? 2 : 1;
+                    g.push_back(f[i]);
+                }
+                start[a].push_back(n[a]);
+                if (pass == 0 && per && any && (g.size() - 1) % 2 == 1) continue;  // odd: leave this axis
+                merged = merged || any;
+                *F[a] = g;
+                break;
             }
-            start[a].push_back(n[a]);
-            *F[a] = g;
         }
         if (!merged) break;
         MGLevel L;
