@@ -126,6 +126,7 @@ struct MGLevel {
     DBuf<int> ci, cj, ck, si, sj, sk, pi, pj, pk;
     DBuf<double> wi, wj, wk;
     DBuf<uint8_t> fmask;               // faces to inactive neighbours (SysCoef::fmask)
+    DBuf<uint8_t> pmask;               // MGTransfer::pm (over the finer level's cells)
     MGTransfer tr{};
     // exact solve of the coarsest level: dense inverse (row-major, rows in
     // x-fastest cell order), padded index and volume of each row's cell

@@ -104,21 +104,25 @@ struct MGTransfer {
     const int *si, *sj, *sk;      // coarse index -> first fine child (n_coarse + 1 entries)
     const int *pi, *pj, *pk;      // fine index -> far coarse neighbour index (or the parent)
     const double *wi, *wj, *wk;   // weight of the parent (1 - weight of the far neighbour)
+    const uint8_t *pm;            // per fine cell: 1 when one of its eight coarse cells is inactive
 };
 
 // r = b - A x on the fluid rows (no norms)
+// returns whether r was set to 0 at the non-fluid cells (else left as it was)
 template <typename T>
-void launch_mg_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
+bool launch_mg_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
                         const T *x, T *r);
 // bc = restriction of the fluid rows' residual b - A x (one pass, no r)
 template <typename T>
 void launch_resid_restrict(hipStream_t st, const Dims &df, const SysCoef<T> &S, const uint8_t *flag_f, const T *b,
                            const T *x, const Dims &dc, const MGTransfer &tr, const double *dxyz_f,
                            const double *dxyz_c, T *bc);
+// rf_zero_off_fluid: rf is 0 at non-fluid cells, so no flags are read
 template <typename T>
 void launch_restrict(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
                      const T *rf, const double *dxf, const double *dyf, const double *dzf, const double *dxc,
-                     const double *dyc, const double *dzc, T *bc);
+                     const double *dyc, const double *dzc, T *bc,
+                     bool rf_zero_off_fluid);
 template <typename T>
 void launch_prolong(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
                     const uint8_t *flag_c, const T *ec, T *xf);

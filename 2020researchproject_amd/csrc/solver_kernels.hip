@@ -369,10 +369,24 @@ __global__ void __launch_bounds__(256) k_mg_residual(Dims d, SysCoef<T> S, const
     r[q] = q == S.pin ? bv - xv : bv - (diag * xv - off);
 }
 
+template <int BY>
+__global__ void __launch_bounds__(64 * BY) k_resid_quad(Dims d, SysCoef<float> S, const uint8_t *flag, const float *b,
+                                                        const float *x, float *r) {
+    resid_quad<BY>(d, S, flag, b, x, r, xcd_block(1));
+}
+
 template <typename T>
-void launch_mg_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
+bool launch_mg_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
                         const T *x, T *r) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (d.nx >= kQuadMinRow) {
+            dim3 grid(((d.nx + 3) / 4 + 63) / 64, (d.ny + 7) / 8, d.nz);
+            hipLaunchKernelGGL((k_resid_quad<8>), grid, dim3(64, 8), 0, st, d, S, flag, b, x, r);
+            return true;
+        }
+    }
     hipLaunchKernelGGL((k_mg_residual<T>), igrid(d), dim3(64, 4), 0, st, d, S, flag, b, x, r);
+    return false;
 }
 
 // Residual of the fluid rows restricted to the coarse grid in one pass:
@@ -725,7 +739,7 @@ void launch_copy(hipStream_t st, const Dims &d, const T *a, T *b) {
 // multigrid transfers
 // ---------------------------------------------------------------------------
 // coarse rhs = volume-weighted sum of the fluid children's residuals / coarse volume
-template <typename T>
+template <typename T, bool FLAGS>
 __global__ void __launch_bounds__(256) k_restrict(Dims df, Dims dc, MGTransfer tr, const uint8_t *flag_f,
                                                   const T *rf, const double *dxf, const double *dyf,
                                                   const double *dzf, const double *dxc, const double *dyc,
@@ -746,7 +760,7 @@ __global__ void __launch_bounds__(256) k_restrict(Dims df, Dims dc, MGTransfer t
     for (int c = 0; c < 8; c++) {
         const long q = cidx(df, ii[c & 1], jj[(c >> 1) & 1], kk[c >> 2]);
         rv[c] = rf[q];
-        fv[c] = flag_f[q];
+        fv[c] = FLAGS ? flag_f[q] : (uint8_t)F_FLUID;  // !FLAGS: rf is 0 at non-fluid cells
     }
     double acc = 0;
 #pragma unroll
@@ -769,13 +783,15 @@ __global__ void __launch_bounds__(256) k_prolong(Dims df, Dims dc, MGTransfer tr
     // the far neighbours are interior coarse cells (the parent itself where
     // the weight is 0 or at non-periodic ends), so all eight loads issue
     // without branches
+    // coarse flags are read only where the stencil touches an inactive cell
+    const bool chk = tr.pm[q] != 0;
     T ev[8];
     uint8_t fv[8];
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const long qc = cidx(dc, Is[c & 1], Js[(c >> 1) & 1], Ks[c >> 2]);
         ev[c] = ec[qc];
-        fv[c] = flag_c[qc];
+        fv[c] = chk ? flag_c[qc] : (uint8_t)F_FLUID;
     }
     const double e0 = (double)ev[0];
     double e = 0;
@@ -863,9 +879,13 @@ __global__ void __launch_bounds__(1024) k_coarse(Dims d, SysCoef<T> S, const uin
 template <typename T>
 void launch_restrict(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
                      const T *rf, const double *dxf, const double *dyf, const double *dzf, const double *dxc,
-                     const double *dyc, const double *dzc, T *bc) {
-    hipLaunchKernelGGL((k_restrict<T>), igrid(dc), dim3(64, 4), 0, st, df, dc, tr, flag_f, rf, dxf, dyf, dzf, dxc, dyc,
-                       dzc, bc);
+                     const double *dyc, const double *dzc, T *bc, bool rf_zero_off_fluid) {
+    if (rf_zero_off_fluid)
+        hipLaunchKernelGGL((k_restrict<T, false>), igrid(dc), dim3(64, 4), 0, st, df, dc, tr, flag_f, rf, dxf, dyf, dzf,
+                           dxc, dyc, dzc, bc);
+    else
+        hipLaunchKernelGGL((k_restrict<T, true>), igrid(dc), dim3(64, 4), 0, st, df, dc, tr, flag_f, rf, dxf, dyf, dzf,
+                           dxc, dyc, dzc, bc);
 }
 template <typename T>
 void launch_prolong(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
@@ -986,7 +1006,7 @@ void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, 
                                        const T *, T *, int);                                                      \
     template void launch_rbgs_color<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *, const T *, T *, \
                                        int, int);                                                                 \
-    template void launch_mg_residual<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *, const T *, \
+    template bool launch_mg_residual<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *, const T *, \
                                         const T *, T *);                                                          \
     template void launch_resid_restrict<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *,         \
                                            const T *, const T *, const Dims &, const MGTransfer &, const double *, \
@@ -1008,7 +1028,7 @@ void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, 
     template void launch_copy<T>(hipStream_t, const Dims &, const T *, T *);                                       \
     template void launch_restrict<T>(hipStream_t, const Dims &, const Dims &, const MGTransfer &, const uint8_t *, \
                                      const T *, const double *, const double *, const double *, const double *,    \
-                                     const double *, const double *, T *);                                         \
+                                     const double *, const double *, T *, bool);                                   \
     template void launch_prolong<T>(hipStream_t, const Dims &, const Dims &, const MGTransfer &, const uint8_t *,  \
                                     const uint8_t *, const T *, T *);                                              \
     template void launch_coarse_solve<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *,           \

@@ -418,8 +418,10 @@ void build_mg(IBMSolver *s) {
         const std::vector<double> *cf[3] = {&P.xc, &P.yc, &P.zc}, *cc[3] = {&L.xc, &L.yc, &L.zc};
         DBuf<int> *pidx[3] = {&L.pi, &L.pj, &L.pk}, *cpar[3] = {&L.ci, &L.cj, &L.ck}, *cst[3] = {&L.si, &L.sj, &L.sk};
         DBuf<double> *pw[3] = {&L.wi, &L.wj, &L.wk};
+        std::vector<int> hfar[3];
         for (int a = 0; a < 3; a++) {
-            std::vector<int> far(n[a]);
+            std::vector<int> &far = hfar[a];
+            far.resize(n[a]);
             std::vector<double> w(n[a]);
             const bool per = s->bc[lo[a]].type == BC_ALL_PERIODIC && s->bc[hi[a]].type == BC_ALL_PERIODIC;
             for (int i = 0; i < n[a]; i++) {
@@ -449,7 +451,23 @@ void build_mg(IBMSolver *s) {
             cpar[a]->upload(parent[a]);
             cst[a]->upload(start[a]);
         }
-        L.tr = {L.ci.p, L.cj.p, L.ck.p, L.si.p, L.sj.p, L.sk.p, L.pi.p, L.pj.p, L.pk.p, L.wi.p, L.wj.p, L.wk.p};
+        {
+            // fine cells whose interpolation stencil touches an inactive coarse cell
+            std::vector<uint8_t> pm(P.d.ncell, 0);
+            for (int k = 0; k < n[2]; k++)
+                for (int j = 0; j < n[1]; j++)
+                    for (int i = 0; i < n[0]; i++) {
+                        const int I2[2] = {parent[0][i], hfar[0][i]}, J2[2] = {parent[1][j], hfar[1][j]},
+                                  K2[2] = {parent[2][k], hfar[2][k]};
+                        bool solid = false;
+                        for (int c = 0; c < 8; c++)
+                            solid |= L.hflag[cidx(L.d, I2[c & 1], J2[(c >> 1) & 1], K2[c >> 2])] == F_SOLID;
+                        pm[cidx(P.d, i, j, k)] = solid ? 1 : 0;
+                    }
+            L.pmask.upload(pm);
+        }
+        L.tr = {L.ci.p, L.cj.p, L.ck.p, L.si.p, L.sj.p, L.sk.p, L.pi.p, L.pj.p, L.pk.p, L.wi.p, L.wj.p, L.wk.p,
+                L.pmask.p};
         L.x64.alloc(L.d.ncell); L.b64.alloc(L.d.ncell); L.r64.alloc(L.d.ncell); L.xa64.alloc(L.d.ncell);
         if (s->precision != IBM3D_PREC_FP64) {
             L.x32.alloc(L.d.ncell); L.b32.alloc(L.d.ncell); L.r32.alloc(L.d.ncell); L.xa32.alloc(L.d.ncell);
@@ -574,12 +592,13 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
         char rn[32], tn[32];
         snprintf(rn, sizeof rn, "mg_resid_L%d", l);
         snprintf(tn, sizeof tn, "mg_restrict_L%d", l);
+        bool rz = false;
         TIMED(rn, 3 * cell_b + cell_b / sizeof(T),
-              launch_mg_residual<T>(s->stream, L.d, S, flag, b, x, lev_r<T>(L).p));
+              rz = launch_mg_residual<T>(s->stream, L.d, S, flag, b, x, lev_r<T>(L).p));
         TIMED(tn, cell_b + cell_b / sizeof(T),
               launch_restrict<T>(s->stream, L.d, Cl.d, Cl.tr, flag, lev_r<T>(L).p, L.dxyz.p, L.dxyz.p + (L.d.nx + 4),
                                  L.dxyz.p + (L.d.nx + 4) + (L.d.ny + 4), Cl.dxyz.p, Cl.dxyz.p + (Cl.d.nx + 4),
-                                 Cl.dxyz.p + (Cl.d.nx + 4) + (Cl.d.ny + 4), lev_b<T>(Cl).p));
+                                 Cl.dxyz.p + (Cl.d.nx + 4) + (Cl.d.ny + 4), lev_b<T>(Cl).p, rz));
     } else {
         // residual restricted in one pass: x and b read, one flag byte per cell
         char nm[32];

@@ -9,7 +9,9 @@
 // AddressSanitizer, every load of the kernel (including the 16-byte loads
 // that run past the last cell of a row) is checked against the allocation.
 // Two launches (colour 0, then colour 1) are compared bit for bit with a
-// direct sweep that applies the stencil cell by cell, as k_rbgs does.
+// direct sweep that applies the stencil cell by cell, as k_rbgs does, and
+// the four-cells-per-lane residual with the one-cell residual at every fluid
+// cell (and with 0 at the others).
 //
 // Usage: rbgs_quad_emul  (prints "ok <cases>" and exits 0 on success)
 #include <cstdio>
@@ -74,6 +76,41 @@ static void run_direct(const Dims &d, const SysCoef<float> &S, const uint8_t *fl
                 }
                 x[q] = diag != 0.f ? (b[q] + off) / diag : 0.f;
             }
+}
+
+static void run_resid_quad(const Dims &d, const SysCoef<float> &S, const uint8_t *flag, const float *b, const float *x,
+                           float *r) {
+    constexpr int BY = 8;  // as launch_mg_residual<float>
+    const int gx = ((d.nx + 3) / 4 + 63) / 64, gy = (d.ny + BY - 1) / BY, gz = d.nz;
+    for (int bz = 0; bz < gz; bz++)
+        for (int by = 0; by < gy; by++)
+            for (int bx = 0; bx < gx; bx++)
+                for (int ty = 0; ty < BY; ty++)
+                    for (int tx = 0; tx < 64; tx++) {
+                        emu_thread = {tx, ty, 0};
+                        resid_quad<BY>(d, S, flag, b, x, r, make_uint3(bx, by, bz));
+                    }
+}
+
+// residual of one fluid cell as the one-cell kernel computes it
+static float direct_resid(const Dims &d, const SysCoef<float> &S, const uint8_t *flag, const float *b, const float *x,
+                          int i, int j, int k) {
+    const int n[3] = {d.nx, d.ny, d.nz};
+    const long st[3] = {1, (long)d.sy, d.sz};
+    const long q = cidx(d, i, j, k);
+    if (q == S.pin) return b[q] - x[q];
+    float diag = S.c0, off = 0;
+    const int id[3] = {i, j, k};
+    for (int a = 0; a < 3; a++) {
+        const AxisCoef<float> &A = S.ax[a];
+        const int ii = id[a];
+        const long qm = (ii == 0 && A.periodic) ? q + (long)(n[a] - 1) * st[a] : q - st[a];
+        const long qp = (ii == n[a] - 1 && A.periodic) ? q - (long)(n[a] - 1) * st[a] : q + st[a];
+        const bool sm = S.skip && (ii > 0 || A.periodic) && flag[qm] == F_SOLID;
+        const bool sp = S.skip && (ii < n[a] - 1 || A.periodic) && flag[qp] == F_SOLID;
+        axis_term(A, ii, sm, sp, x[qm], x[qp], diag, off);
+    }
+    return b[q] - (diag * x[q] - off);
 }
 
 static bool one_case(std::mt19937_64 &rng, int nx, int ny, int nz, int per, int skip, int pin, int zero_in) {
@@ -141,6 +178,20 @@ static bool one_case(std::mt19937_64 &rng, int nx, int ny, int nz, int per, int 
                     if (bad < 5)
                         fprintf(stderr, "mismatch (%d,%d,%d): quad %.9g direct %.9g\n", i, j, k, (double)xq[q],
                                 (double)xd[q]);
+                    bad++;
+                }
+            }
+    // residual of the swept field at the fluid cells
+    std::vector<float> r(d.ncell, 0.f);
+    std::fill(r.begin(), r.end(), -3.f);
+    run_resid_quad(d, S, flag.data(), b.data(), xq.data(), r.data());
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const long q = cidx(d, i, j, k);
+                const float want = flag[q] != F_FLUID ? 0.f : direct_resid(d, S, flag.data(), b.data(), xq.data(), i, j, k);
+                if (std::memcmp(&r[q], &want, sizeof(float)) != 0) {
+                    if (bad < 5) fprintf(stderr, "residual mismatch (%d,%d,%d): %.9g vs %.9g\n", i, j, k, r[q], want);
                     bad++;
                 }
             }
