@@ -565,6 +565,7 @@ static void alloc_fields(IBMSolver *s) {
     }
     long nb = conv_rhs_blocks(s->d) + 64;
     nb = std::max(nb, apply_partials(s->d, s->ng) + 64);
+    nb = std::max(nb, vec_partials(s->d) + 64);
     s->partial.alloc((size_t)nb * 4);
 }
 
