@@ -64,6 +64,7 @@ def run_gpu(prec: int, nx: int, ny: int, nz: int, steps: int, warmup: int, tol: 
         "u": [s.get(n)[2:-2, 2:-2, 2:-2] for n in ("U1", "U2", "U3")],
         "p": s.get("P")[2:-2, 2:-2, 2:-2],
         "force": s.body_force().tolist(),
+        "mg_levels": ["x".join(str(v) for v in lv) for lv in s.mg_levels()],
     }
     log(f"[bench] prec={prec} timed {steps} steps: {wall:.3f} s, max|div U| {res['max_div']:.3e}, "
         f"its {res['iters']}")
@@ -246,7 +247,8 @@ def main() -> int:
         "config": {"workload": f"sphere {nx}x{ny}x{nz} cosine-stretched, Re 300, dt 0.002, tol {args.tol:g}",
                    "cells": int(cells),
                    "precision": ("FP64 BiCGSTAB, FP32 multigrid preconditioner" if args.mixed_mode == 2 else
-                                 "FP64 defect correction around FP32 multigrid-BiCGSTAB")},
+                                 "FP64 defect correction around FP32 multigrid-BiCGSTAB"),
+                   "mg_levels": mixed["mg_levels"]},
         "p50_ms": round(float(np.percentile(mixed["step_ms"], 50)), 3),
         "p99_ms": round(float(np.percentile(mixed["step_ms"], 99)), 3),
         "max_div": mixed["max_div"],
