@@ -136,7 +136,8 @@ CAVITY_SEED = 2020
 
 def setup_cavity(s: I.Solver, n: int = 32, Re: float = 100.0, dt: float = 0.005,
                  tol: float = 1e-10, seed: int = CAVITY_SEED, amp: float = 1e-3,
-                 precision: int = I.PREC_FP64, init: str = "array", shape=None):
+                 precision: int = I.PREC_FP64, init: str = "array", shape=None,
+                 solver: int = I.SOLVER_BiCGSTAB, precond: int = I.PRECOND_AMG):
     """Lid-driven cavity on the unit cube, n^3 cells (or shape = (nx, ny, nz))."""
     nx, ny, nz = shape if shape is not None else (n, n, n)
     fx, fy, fz = uniform_faces(0.0, 1.0, nx), uniform_faces(0.0, 1.0, ny), uniform_faces(0.0, 1.0, nz)
@@ -145,7 +146,7 @@ def setup_cavity(s: I.Solver, n: int = 32, Re: float = 100.0, dt: float = 0.005,
     s.set_bc(I.DIR_NORTH, I.BC_VELOCITY_COMPONENT, I.VAL_CONST, 1.0, 0.0, 0.0)
     s.set_bc(I.DIR_EAST | I.DIR_WEST | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP, I.BC_STATIONARY_WALL)
     s.set_obstacle(None)
-    s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, tol)
+    s.set_linear_solver(solver, precond, tol)
     s.set_precision(precision)
     s.assemble()
 
@@ -200,7 +201,8 @@ def sphere_faces(nx: int = 256, ny: int = 128, nz: int = 128, dmin: float = SPHE
 
 def setup_sphere(s: I.Solver, nx: int = 256, ny: int = 128, nz: int = 128, Re: float = 300.0,
                  dt: float = 0.002, tol: float = 1e-10, precision: int = I.PREC_FP64,
-                 dmin: float | None = None, faces=None):
+                 dmin: float | None = None, faces=None, solver: int = I.SOLVER_BiCGSTAB,
+                 precond: int = I.PRECOND_AMG):
     if faces is None:
         if dmin is None:
             # keep the cell-width ratio of the 256x128x128 grid on coarser grids
@@ -213,7 +215,7 @@ def setup_sphere(s: I.Solver, nx: int = 256, ny: int = 128, nz: int = 128, Re: f
     s.set_bc(I.DIR_EAST, I.BC_PRESSURE, I.VAL_CONST, 0.0)
     s.set_bc(I.DIR_NORTH | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP, I.BC_FREE_SLIP_WALL)
     s.set_obstacle_sphere(0.0, 0.0, 0.0, 0.5)
-    s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, tol)
+    s.set_linear_solver(solver, precond, tol)
     s.set_precision(precision)
     s.assemble()
     s.init_flow_const(1.0, 0.0, 0.0, 0.0)

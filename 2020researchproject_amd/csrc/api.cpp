@@ -54,6 +54,7 @@ IBMSolver *IBMSolver_new(const int num_process, const int rank, const int Px, co
     HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_CHECK(hipHostMalloc((void **)&s->ks_h, sizeof(KScal), hipHostMallocDefault));
     HIP_CHECK(hipHostMalloc((void **)&s->sums_h, kSumSlots * sizeof(double), hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc((void **)&s->gm_h, 64 * sizeof(double), hipHostMallocDefault));
     s->ks.alloc(1);
     s->sums.alloc(kSumSlots);
     return s;
@@ -69,6 +70,7 @@ void IBMSolver_destroy(IBMSolver *s) {
     for (auto &g : s->vgraphs) (void)hipGraphExecDestroy(g.exec);
     (void)hipHostFree(s->ks_h);
     (void)hipHostFree(s->sums_h);
+    (void)hipHostFree(s->gm_h);
     (void)hipStreamDestroy(s->stream);
     delete s;  // device buffers are released by their owners' destructors
 }
@@ -563,6 +565,18 @@ static void alloc_fields(IBMSolver *s) {
         s->ref_rhs32.alloc(n);
         s->ref_e32.alloc(n);
     }
+    if (s->solver_type != SOLVER_BiCGSTAB) {
+        // pressure solver scratch: GMRES basis, or the PCG / stationary
+        // multigrid vectors (Krylov vectors above) and scalars
+        GmresData &G = s->gm;
+        if (s->solver_type == SOLVER_GMRES) {
+            for (int v = 0; v <= kGmresRestart; v++) G.V[v].alloc(n);
+            for (int v = 0; v < kGmresRestart; v++) G.Z[v].alloc(n);
+            G.t.alloc(n);
+        }
+        G.dev.alloc(64);
+        G.partial.alloc((size_t)krylov_blocks(s->d) * (kVecSetMax + 1));
+    }
     long nb = conv_rhs_blocks(s->d) + 64;
     nb = std::max(nb, apply_partials(s->d, s->ng) + 64);
     nb = std::max(nb, vec_partials(s->d) + 64);
@@ -578,6 +592,20 @@ extern "C" void IBMSolver_assemble(IBMSolver *s) {
     if (isperiodic(s->bc[0].type) != isperiodic(s->bc[2].type) || isperiodic(s->bc[1].type) != isperiodic(s->bc[3].type)
         || isperiodic(s->bc[4].type) != isperiodic(s->bc[5].type)) {
         printf("Inconsistent periodic boundary condition\n");
+        abort();
+    }
+    // pressure solver and preconditioner types  <- src/ibm3d_setup.c:1435-1536
+    if (s->solver_type != SOLVER_AMG && s->solver_type != SOLVER_PCG && s->solver_type != SOLVER_BiCGSTAB &&
+        s->solver_type != SOLVER_GMRES) {
+        printf("\nUnknown linear solver type\n");
+        abort();
+    }
+    if (s->precond_type != PRECOND_NONE && s->precond_type != PRECOND_AMG) {
+        printf("\nUnknown preconditioner type\n");
+        abort();
+    }
+    if (s->solver_type == SOLVER_AMG && s->precond_type != PRECOND_NONE) {
+        printf("\nCannot use preconditioner with BoomerAMG solver\n");
         abort();
     }
     s->p_singular = true;  // <- src/ibm3d_setup.c:1598-1603

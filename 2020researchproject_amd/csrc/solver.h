@@ -155,6 +155,15 @@ struct Krylov {
     DBuf<float> v32[8];
 };
 
+// Restarted GMRES storage (pressure, SOLVER_GMRES): the Arnoldi basis V,
+// the preconditioned directions Z, one unscaled vector, device scalars and
+// the block partial sums of the multi-dot kernels.
+constexpr int kGmresRestart = 10;  // Krylov dimension: src/ibm3d_setup.c:1474
+struct GmresData {
+    DBuf<double> V[kGmresRestart + 1], Z[kGmresRestart], t;
+    DBuf<double> dev, partial;  // dev: 64 scalars
+};
+
 }  // namespace ibm
 
 struct _ibm_solver {
@@ -216,6 +225,7 @@ struct _ibm_solver {
     ibm::DBuf<ibm::KScal> ks;
     ibm::KScal *ks_h = nullptr;      // pinned
     double *sums_h = nullptr;        // pinned
+    double *gm_h = nullptr;          // pinned, 64 scalars of the GMRES / PCG solvers
 
     // ghost cells
     int ng = 0;
@@ -232,6 +242,7 @@ struct _ibm_solver {
     std::vector<ibm::MGLevel> mg;         // mg[0] = fine pressure level
     std::vector<ibm::VGraph> vgraphs;     // captured V-cycles per (input, output)
     ibm::Krylov kry;
+    ibm::GmresData gm;                    // SOLVER_GMRES / SOLVER_PCG / SOLVER_AMG scratch (pressure)
     ibm::DBuf<double> ref_r64;            // mixed mode: FP64 defect
     ibm::DBuf<float> ref_rhs32, ref_e32;  // mixed mode: FP32 defect and correction
 

@@ -80,6 +80,37 @@ void launch_bicg_xr(hipStream_t st, const Dims &d, const KScal *ks, const T *ph,
                     const T *t, const T *rh, T *x, T *r, const NormSpec &ns, double *partial);
 long vec_partials(const Dims &d);
 
+// Restarted GMRES, PCG and stationary multigrid: up to kVecSetMax vectors
+// per multi-dot / multi-update launch.
+constexpr int kVecSetMax = 12;
+template <typename T>
+struct VecSet {
+    const T *v[kVecSetMax];
+};
+// partial-sum blocks of the kernels below (scratch of kVecSetMax + 1 values each)
+long krylov_blocks(const Dims &d);
+// out[a] = (w, V_a) for a < nv, out[kVecSetMax] = (w, w); scale: w <- N w
+// first (N: the normalised-row scale of ns)
+template <typename T>
+void launch_gm_dots(hipStream_t st, const Dims &d, const VecSet<T> &V, int nv, T *w, const NormSpec &ns, int scale,
+                    double *partial, double *out);
+// w <- w - sum_{a < nv} c_a V_a (c on the device); out[0] = (w, w)
+template <typename T>
+void launch_gm_update(hipStream_t st, const Dims &d, const VecSet<T> &V, int nv, const double *c, T *w,
+                      double *partial, double *out);
+// v = f w, or (unscale) v = w / nscale
+template <typename T>
+void launch_gm_scale(hipStream_t st, const Dims &d, const T *w, double f, T *v, const NormSpec &ns, int unscale);
+// x += a p, r -= a q; out = (r, r), (N r, N r)
+template <typename T>
+void launch_pcg_update(hipStream_t st, const Dims &d, double a, const T *p, const T *q, T *x, T *r,
+                       const NormSpec &ns, double *partial, double *out);
+template <typename T>
+void launch_dot(hipStream_t st, const Dims &d, const T *a, const T *b, double *partial, double *out);
+// y = z + f y, or (axpy) y = y + f z
+template <typename T>
+void launch_lincomb(hipStream_t st, const Dims &d, const T *z, double f, T *y, int axpy);
+
 // finalize ops: 0 = alpha from dot(rh, v); 1 = omega from (t.s, t.t);
 // 2 = rho/beta/norms after the x, r update; 3 = store b norms and rho1 at
 // start; 4 = store residual norms only

@@ -786,6 +786,176 @@ void launch_finalize(hipStream_t st, int op, const double *partial, long nblk, i
 }
 
 // ---------------------------------------------------------------------------
+// Vector kernels of the restarted GMRES (Saad & Schultz 1986, classical
+// Gram-Schmidt applied twice), preconditioned CG and stationary multigrid
+// pressure solvers. Sums are block partials added in a fixed order.
+// ---------------------------------------------------------------------------
+// 1 / nscale: the row's sum of K for fluid rows, else 1
+__device__ inline double nfactor(const NormSpec &ns, int i, int j, int k, long q) {
+    if (!ns.on || q == ns.pin || ns.flag[q] != F_FLUID) return 1.0;
+    return ns.ksx[i] + ns.ksy[j] + ns.ksz[k];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_gm_dots(Dims d, VecSet<T> V, int nv, T *w, NormSpec ns, int scale,
+                                                 double *partial) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * kRedZ, k1 = min(k0 + kRedZ, d.nz);
+    double acc[kVecSetMax + 1];
+#pragma unroll
+    for (int a = 0; a <= kVecSetMax; a++) acc[a] = 0;
+    if (i < d.nx && j < d.ny)
+        for (int k = k0; k < k1; k++) {
+            const long q = cidx(d, i, j, k);
+            T wv = w[q];
+            if (scale) {
+                wv = (T)((double)wv * nscale(ns, d, i, j, k, q));
+                w[q] = wv;
+            }
+#pragma unroll
+            for (int a = 0; a < kVecSetMax; a++)
+                if (a < nv) acc[a] += (double)wv * (double)V.v[a][q];
+            acc[kVecSetMax] += (double)wv * (double)wv;
+        }
+    bsum<kVecSetMax + 1>(acc, partial, blin());
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_gm_update(Dims d, VecSet<T> V, int nv, const double *c, T *w,
+                                                   double *partial) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * kRedZ, k1 = min(k0 + kRedZ, d.nz);
+    double cv[kVecSetMax];
+#pragma unroll
+    for (int a = 0; a < kVecSetMax; a++) cv[a] = a < nv ? c[a] : 0.0;
+    double acc[1] = {0};
+    if (i < d.nx && j < d.ny)
+        for (int k = k0; k < k1; k++) {
+            const long q = cidx(d, i, j, k);
+            double sum = 0;
+#pragma unroll
+            for (int a = 0; a < kVecSetMax; a++)
+                if (a < nv) sum += cv[a] * (double)V.v[a][q];
+            const T wv = (T)((double)w[q] - sum);
+            w[q] = wv;
+            acc[0] += (double)wv * (double)wv;
+        }
+    bsum<1>(acc, partial, blin());
+}
+
+// v = f w (f on the host); unscale: t = v / nscale (the normalised rows' scale undone)
+template <typename T>
+__global__ void __launch_bounds__(256) k_gm_scale(Dims d, const T *w, double f, T *v, NormSpec ns, int unscale) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    v[q] = (T)((double)w[q] * (unscale ? nfactor(ns, i, j, k, q) : f));
+}
+
+// x += a p, r -= a q; sums of r^2 and (N r)^2
+template <typename T>
+__global__ void __launch_bounds__(256) k_pcg_update(Dims d, double a, const T *p, const T *qv, T *x, T *r,
+                                                    NormSpec ns, double *partial) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * kRedZ, k1 = min(k0 + kRedZ, d.nz);
+    double acc[2] = {0, 0};
+    if (i < d.nx && j < d.ny)
+        for (int k = k0; k < k1; k++) {
+            const long q = cidx(d, i, j, k);
+            x[q] = (T)((double)x[q] + a * (double)p[q]);
+            const T rv = (T)((double)r[q] - a * (double)qv[q]);
+            r[q] = rv;
+            acc[0] += (double)rv * (double)rv;
+            const double rs = (double)rv * nscale(ns, d, i, j, k, q);
+            acc[1] += rs * rs;
+        }
+    bsum<2>(acc, partial, blin());
+}
+
+// (a, b) per block
+template <typename T>
+__global__ void __launch_bounds__(256) k_dot(Dims d, const T *a, const T *b, double *partial) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * kRedZ, k1 = min(k0 + kRedZ, d.nz);
+    double acc[1] = {0};
+    if (i < d.nx && j < d.ny)
+        for (int k = k0; k < k1; k++) {
+            const long q = cidx(d, i, j, k);
+            acc[0] += (double)a[q] * (double)b[q];
+        }
+    bsum<1>(acc, partial, blin());
+}
+
+// y = z + f y (xpby) or y = y + f z (axpy)
+template <typename T>
+__global__ void __launch_bounds__(256) k_lincomb(Dims d, const T *z, double f, T *y, int axpy) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long q = cidx(d, i, j, k);
+    y[q] = axpy ? (T)((double)y[q] + f * (double)z[q]) : (T)((double)z[q] + f * (double)y[q]);
+}
+
+// out[a] = sum over blocks of partial[b * nv + a], one workgroup per column
+__global__ void __launch_bounds__(kFinThreads) k_sum_cols(const double *partial, long nblk, int nv, double *out) {
+    __shared__ double sh[kFinThreads];
+    const int a = blockIdx.x, t = threadIdx.x;
+    double acc = 0;
+    for (long b = t; b < nblk; b += kFinThreads) acc += partial[b * nv + a];
+    sh[t] = acc;
+    __syncthreads();
+    for (int w = kFinThreads / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[a] = sh[0];
+}
+
+long krylov_blocks(const Dims &d) { return iblocks(d); }
+
+template <typename T>
+void launch_gm_dots(hipStream_t st, const Dims &d, const VecSet<T> &V, int nv, T *w, const NormSpec &ns, int scale,
+                    double *partial, double *out) {
+    hipLaunchKernelGGL((k_gm_dots<T>), rgrid(d), dim3(64, 4), 0, st, d, V, nv, w, ns, scale, partial);
+    hipLaunchKernelGGL(k_sum_cols, dim3(kVecSetMax + 1), dim3(kFinThreads), 0, st, partial, iblocks(d), kVecSetMax + 1,
+                       out);
+}
+template <typename T>
+void launch_gm_update(hipStream_t st, const Dims &d, const VecSet<T> &V, int nv, const double *c, T *w,
+                      double *partial, double *out) {
+    hipLaunchKernelGGL((k_gm_update<T>), rgrid(d), dim3(64, 4), 0, st, d, V, nv, c, w, partial);
+    hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(kFinThreads), 0, st, partial, iblocks(d), 1, out);
+}
+template <typename T>
+void launch_gm_scale(hipStream_t st, const Dims &d, const T *w, double f, T *v, const NormSpec &ns, int unscale) {
+    hipLaunchKernelGGL((k_gm_scale<T>), igrid(d), dim3(64, 4), 0, st, d, w, f, v, ns, unscale);
+}
+template <typename T>
+void launch_pcg_update(hipStream_t st, const Dims &d, double a, const T *p, const T *q, T *x, T *r,
+                       const NormSpec &ns, double *partial, double *out) {
+    hipLaunchKernelGGL((k_pcg_update<T>), rgrid(d), dim3(64, 4), 0, st, d, a, p, q, x, r, ns, partial);
+    hipLaunchKernelGGL(k_sum_cols, dim3(2), dim3(kFinThreads), 0, st, partial, iblocks(d), 2, out);
+}
+template <typename T>
+void launch_dot(hipStream_t st, const Dims &d, const T *a, const T *b, double *partial, double *out) {
+    hipLaunchKernelGGL((k_dot<T>), rgrid(d), dim3(64, 4), 0, st, d, a, b, partial);
+    hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(kFinThreads), 0, st, partial, iblocks(d), 1, out);
+}
+template <typename T>
+void launch_lincomb(hipStream_t st, const Dims &d, const T *z, double f, T *y, int axpy) {
+    hipLaunchKernelGGL((k_lincomb<T>), igrid(d), dim3(64, 4), 0, st, d, z, f, y, axpy);
+}
+template void launch_gm_dots<double>(hipStream_t, const Dims &, const VecSet<double> &, int, double *,
+                                     const NormSpec &, int, double *, double *);
+template void launch_gm_update<double>(hipStream_t, const Dims &, const VecSet<double> &, int, const double *,
+                                       double *, double *, double *);
+template void launch_gm_scale<double>(hipStream_t, const Dims &, const double *, double, double *, const NormSpec &,
+                                      int);
+template void launch_pcg_update<double>(hipStream_t, const Dims &, double, const double *, const double *, double *,
+                                        double *, const NormSpec &, double *, double *);
+template void launch_dot<double>(hipStream_t, const Dims &, const double *, const double *, double *, double *);
+template void launch_lincomb<double>(hipStream_t, const Dims &, const double *, double, double *, int);
+
+// ---------------------------------------------------------------------------
 // precision conversion and copies over interior cells
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_d2f(Dims d, const double *a, float *b) {

@@ -755,6 +755,188 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
     }
 }
 
+// ---------------------------------------------------------------------------
+// The other pressure solver types of src/ibm3d_setup.c:1434-1536 (FP64
+// iterates; the multigrid cycle runs in the precision the mode selects).
+// `prec` as for bicgstab: 0 none, 1 Jacobi, 2 multigrid, 3 FP32 multigrid.
+// ---------------------------------------------------------------------------
+static void read_gm(IBMSolver *s, int n) {
+    HIP_CHECK(hipMemcpyAsync(s->gm_h, s->gm.dev.p, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+}
+
+static void precond64(IBMSolver *s, int prec, const double *in, double *out) {
+    const double cb = ncells(s) * sizeof(double);
+    if (prec == 1) TIMED("jacobi", 2 * cb, launch_jacobi<double>(s->stream, s->d, s->sys[3].S64, s->gp.G64, s->flag.p, in, out));
+    else if (prec == 2) mg_precond<double>(s, in, out);
+    else if (prec == 3) mg_precond_f32(s, in, out);
+    else launch_copy<double>(s->stream, s->d, in, out);
+}
+
+// true residual r = b - A x; returns its norm (normalised rows when ns.on)
+static double true_residual(IBMSolver *s, const double *b, const double *x, double *r, const NormSpec &ns) {
+    const Dims &d = s->d;
+    TIMED("residual", 3 * ncells(s) * 8,
+          launch_residual<double>(s->stream, d, s->sys[3].S64, s->gp.G64, s->flag.p, b, x, r, ns, s->partial.p));
+    launch_finalize(s->stream, 4, s->partial.p, apply_partials(d, s->gp.G64.ng), 2, s->ks.p);
+    read_ks(s);
+    return std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
+}
+
+// Restarted GMRES(10) (Saad & Schultz 1986) on the system with normalised
+// rows (each fluid row divided by its sum of K, the system whose residual
+// the reference's stopping test measures), right-preconditioned by the
+// multigrid cycle applied after undoing that scaling, so that the
+// preconditioned operator stays close to the identity. Arnoldi with
+// classical Gram-Schmidt applied twice (one host read per step), Givens
+// rotations on the host; x0 = 0 as in src/ibm3d_fracstep.c:650.
+static int gmres_p(IBMSolver *s, const double *b, double *x, int prec, const NormSpec &ns, double tol, int maxit,
+                   double bn, double *final_rel) {
+    const Dims &d = s->d;
+    GmresData &W = s->gm;
+    const int m = kGmresRestart;
+    const double cb = ncells(s) * sizeof(double);
+    double *dev = W.dev.p, *part = W.partial.p, *h = s->gm_h;
+    launch_zero<double>(s->stream, d, x);
+    auto residual_to_v0 = [&]() {
+        // V0 = N (b - A x), returns |V0|
+        true_residual(s, b, x, W.V[0].p, ns);
+        VecSet<double> none{};
+        TIMED("gmres_dots", 2 * cb, launch_gm_dots<double>(s->stream, d, none, 0, W.V[0].p, ns, 1, part, dev));
+        read_gm(s, kVecSetMax + 1);
+        return std::sqrt(h[kVecSetMax]);
+    };
+    double beta = residual_to_v0(), rn = beta;
+    if (bn == 0) { *final_rel = 0; return 0; }
+    int it = 0;
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    auto Hc = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+    while (it < maxit && rn > tol * bn && std::isfinite(rn)) {
+        launch_gm_scale<double>(s->stream, d, W.V[0].p, 1.0 / beta, W.V[0].p, ns, 0);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        while (j < m && it < maxit) {
+            it++;
+            launch_gm_scale<double>(s->stream, d, W.V[j].p, 1.0, W.t.p, ns, 1);
+            precond64(s, prec, W.t.p, W.Z[j].p);
+            TIMED("spmv_p64", 3 * cb,
+                  launch_apply<double>(s->stream, d, s->sys[3].S64, s->gp.G64, s->flag.p, W.Z[j].p, W.V[j + 1].p, 0,
+                                       nullptr, s->partial.p));
+            VecSet<double> Vs{};
+            for (int i = 0; i <= j; i++) Vs.v[i] = W.V[i].p;
+            TIMED("gmres_dots", (j + 3) * cb,
+                  launch_gm_dots<double>(s->stream, d, Vs, j + 1, W.V[j + 1].p, ns, 1, part, dev));
+            TIMED("gmres_update", (j + 3) * cb,
+                  launch_gm_update<double>(s->stream, d, Vs, j + 1, dev, W.V[j + 1].p, part, dev + 16));
+            TIMED("gmres_dots", (j + 2) * cb,
+                  launch_gm_dots<double>(s->stream, d, Vs, j + 1, W.V[j + 1].p, ns, 0, part, dev + 32));
+            TIMED("gmres_update", (j + 3) * cb,
+                  launch_gm_update<double>(s->stream, d, Vs, j + 1, dev + 32, W.V[j + 1].p, part, dev + 48));
+            read_gm(s, 49);
+            const double hn = std::sqrt(std::max(h[48], 0.0));
+            for (int i = 0; i <= j; i++) Hc(i, j) = h[i] + h[32 + i];
+            Hc(j + 1, j) = hn;
+            if (hn > 0) launch_gm_scale<double>(s->stream, d, W.V[j + 1].p, 1.0 / hn, W.V[j + 1].p, ns, 0);
+            for (int i = 0; i < j; i++) {
+                const double t0 = cs[i] * Hc(i, j) + sn[i] * Hc(i + 1, j);
+                Hc(i + 1, j) = -sn[i] * Hc(i, j) + cs[i] * Hc(i + 1, j);
+                Hc(i, j) = t0;
+            }
+            const double den = std::hypot(Hc(j, j), Hc(j + 1, j));
+            cs[j] = den > 0 ? Hc(j, j) / den : 1.0;
+            sn[j] = den > 0 ? Hc(j + 1, j) / den : 0.0;
+            Hc(j, j) = den;
+            Hc(j + 1, j) = 0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            rn = std::fabs(g[j + 1]);
+            j++;
+            if (!std::isfinite(rn) || rn <= tol * bn || hn == 0) break;
+        }
+        // x += Z y, H(0:j, 0:j) y = g(0:j)
+        for (int i = j - 1; i >= 0; i--) {
+            double acc = g[i];
+            for (int c = i + 1; c < j; c++) acc -= Hc(i, c) * y[c];
+            y[i] = Hc(i, i) != 0 ? acc / Hc(i, i) : 0.0;
+        }
+        for (int i = 0; i < j; i++) h[i] = -y[i];
+        HIP_CHECK(hipMemcpyAsync(dev, h, j * sizeof(double), hipMemcpyHostToDevice, s->stream));
+        VecSet<double> Zs{};
+        for (int i = 0; i < j; i++) Zs.v[i] = W.Z[i].p;
+        TIMED("gmres_update", (j + 2) * cb, launch_gm_update<double>(s->stream, d, Zs, j, dev, x, part, dev + 16));
+        beta = residual_to_v0();
+        rn = beta;
+    }
+    *final_rel = rn / bn;
+    return it;
+}
+
+// Preconditioned conjugate gradients (src/ibm3d_setup.c:1459-1464): the
+// reference offers it for the pressure although the ghost rows make the
+// operator nonsymmetric. Stops on the true residual's normalised norm.
+static int pcg_p(IBMSolver *s, const double *b, double *x, int prec, const NormSpec &ns, double tol, int maxit,
+                 double bn, double *final_rel) {
+    const Dims &d = s->d;
+    DBuf<double> *K = s->kry.v64;
+    double *r = K[0].p, *z = K[1].p, *pv = K[2].p, *q = K[3].p;
+    double *dev = s->gm.dev.p, *part = s->gm.partial.p, *h = s->gm_h;
+    const double cb = ncells(s) * sizeof(double);
+    launch_zero<double>(s->stream, d, x);
+    double rn = true_residual(s, b, x, r, ns);
+    if (bn == 0) { *final_rel = 0; return 0; }
+    precond64(s, prec, r, z);
+    launch_dot<double>(s->stream, d, r, z, part, dev);
+    read_gm(s, 1);
+    double rho = h[0];
+    launch_copy<double>(s->stream, d, z, pv);
+    int it = 0;
+    while (it < maxit && rn > tol * bn && std::isfinite(rn)) {
+        it++;
+        TIMED("spmv_p64", 3 * cb,
+              launch_apply<double>(s->stream, d, s->sys[3].S64, s->gp.G64, s->flag.p, pv, q, 0, nullptr, s->partial.p));
+        launch_dot<double>(s->stream, d, pv, q, part, dev);
+        read_gm(s, 1);
+        if (h[0] == 0 || !std::isfinite(h[0])) break;
+        const double alpha = rho / h[0];
+        TIMED("pcg_update", 5 * cb, launch_pcg_update<double>(s->stream, d, alpha, pv, q, x, r, ns, part, dev));
+        read_gm(s, 2);
+        rn = std::sqrt(ns.on ? h[1] : h[0]);
+        if (rn <= tol * bn) break;
+        precond64(s, prec, r, z);
+        launch_dot<double>(s->stream, d, r, z, part, dev);
+        read_gm(s, 1);
+        const double rho1 = h[0];
+        if (rho == 0 || !std::isfinite(rho1)) break;
+        launch_lincomb<double>(s->stream, d, z, rho1 / rho, pv, 0);
+        rho = rho1;
+    }
+    rn = true_residual(s, b, x, r, ns);
+    *final_rel = rn / bn;
+    return it;
+}
+
+// The multigrid cycle as a stationary solver (BoomerAMG as the solver,
+// src/ibm3d_setup.c:1435-1457): x += M (b - A x) until the residual test.
+static int mg_solve_p(IBMSolver *s, const double *b, double *x, int prec, const NormSpec &ns, double tol, int maxit,
+                      double bn, double *final_rel) {
+    const Dims &d = s->d;
+    DBuf<double> *K = s->kry.v64;
+    double *r = K[0].p, *z = K[1].p;
+    launch_zero<double>(s->stream, d, x);
+    double rn = true_residual(s, b, x, r, ns);
+    if (bn == 0) { *final_rel = 0; return 0; }
+    int it = 0;
+    while (it < maxit && rn > tol * bn && std::isfinite(rn)) {
+        it++;
+        precond64(s, prec, r, z);
+        launch_lincomb<double>(s->stream, d, z, 1.0, x, 1);
+        rn = true_residual(s, b, x, r, ns);
+    }
+    *final_rel = rn / bn;
+    return it;
+}
+
 // FP64 iterative refinement around FP32 BiCGSTAB inner solves.
 static int refine_mixed(IBMSolver *s, int sk, const double *b, double *x, int prec, const NormSpec &ns, double tol,
                         int maxit, double bnorm, double *final_rel) {
@@ -807,7 +989,15 @@ static void solve_system(IBMSolver *s, int sk, const double *b, double *x, doubl
     const int maxit = sk == 3 ? 1000 : 200;  // src/ibm3d_setup.c:1413, 1467
     double rel = 0;
     int it;
-    if (s->precision == IBM3D_PREC_MIXED) {
+    if (sk == 3 && s->solver_type != SOLVER_BiCGSTAB && s->precision != IBM3D_PREC_MIXED) {
+        // multigrid cycles in FP32 in the mixed-Krylov mode; SOLVER_AMG is the
+        // cycle itself, with no separate preconditioner
+        const int pp = s->solver_type == SOLVER_AMG ? 2 : prec;
+        const int p2 = pp == 2 && s->precision == IBM3D_PREC_MIXED_KRYLOV ? 3 : pp;
+        if (s->solver_type == SOLVER_GMRES) it = gmres_p(s, b, x, p2, ns, s->tol, maxit, bnorm, &rel);
+        else if (s->solver_type == SOLVER_PCG) it = pcg_p(s, b, x, p2, ns, s->tol, maxit, bnorm, &rel);
+        else it = mg_solve_p(s, b, x, p2, ns, s->tol, maxit, bnorm, &rel);
+    } else if (s->precision == IBM3D_PREC_MIXED) {
         it = refine_mixed(s, sk, b, x, prec, ns, s->tol, maxit, bnorm, &rel);
     } else if (s->precision == IBM3D_PREC_MIXED_KRYLOV) {
         const GhostData &GD = sk == 3 ? s->gp : s->gmom;

@@ -317,3 +317,57 @@ def test_divergence_bound_at_bench_tolerance(new_gpu, cases, precision):
     g.iterate(3)
     assert g.max_div() <= 1e-8
     assert (g.last_norms() <= 1e-9).all()
+
+
+# ---------------------------------------------------------------------------
+# The reference's other pressure solver types (src/ibm3d_setup.c:1434-1536):
+# the solution of each step is fixed by the tolerance, so every type matches
+# the oracle's BiCGSTAB to the same bar.
+# ---------------------------------------------------------------------------
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver,precision", [("GMRES", "FP64"), ("GMRES", "MIXED_KRYLOV"), ("AMG", "FP64"),
+                                              ("AMG", "MIXED_KRYLOV")])
+def test_sphere_pressure_solver_types(new_gpu, new_oracle, cases, I, solver, precision):
+    st = getattr(I, "SOLVER_" + solver)
+    pc = I.PRECOND_NONE if solver == "AMG" else I.PRECOND_AMG
+    g, o = new_gpu(), new_oracle()
+    cases.setup_sphere(g, 48, 24, 24, tol=1e-11, precision=getattr(I, "PREC_" + precision), solver=st, precond=pc)
+    cases.setup_sphere(o, 48, 24, 24, tol=1e-12)
+    g.iterate(4)
+    o.iterate(4)
+    compare(g, o, 1e-7)
+    assert g.max_div() < 1e-8
+    assert g.last_norms()[3] <= 1e-11
+
+
+def setup_open_box(s, I, cases, solver, precond, n=(24, 12, 12), tol=1e-12):
+    """Uniform grid, velocity inlet, pressure outlet, free-slip sides, no
+    body: the pressure operator is symmetric and not pinned."""
+    s.set_grid(cases.uniform_faces(0.0, 2.0, n[0]), cases.uniform_faces(0.0, 1.0, n[1]),
+               cases.uniform_faces(0.0, 1.0, n[2]))
+    s.set_params(100.0, 0.01)
+    s.set_bc(I.DIR_WEST, I.BC_VELOCITY_COMPONENT, I.VAL_CONST, 1.0, 0.0, 0.0)
+    s.set_bc(I.DIR_EAST, I.BC_PRESSURE, I.VAL_CONST, 0.0)
+    s.set_bc(I.DIR_NORTH | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP, I.BC_FREE_SLIP_WALL)
+    s.set_obstacle(None)
+    s.set_linear_solver(solver, precond, tol)
+    s.assemble()
+    s.init_flow_func(lambda x, y, z: 1.0 + 0.2 * np.sin(3 * y) * np.cos(2 * z),
+                     lambda x, y, z: 0.1 * np.sin(2 * x), lambda x, y, z: 0.05 * np.cos(3 * x) * np.sin(2 * y),
+                     lambda x, y, z: 0.0)
+    return s
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver", ["PCG", "GMRES", "AMG"])
+def test_open_box_pressure_solver_types(new_gpu, new_oracle, cases, I, solver):
+    """PCG needs a symmetric operator: a uniform grid without a body or a
+    pinned cell; GMRES and the stationary cycle run on the same case."""
+    st = getattr(I, "SOLVER_" + solver)
+    pc = I.PRECOND_NONE if solver == "AMG" else I.PRECOND_AMG
+    g = setup_open_box(new_gpu(), I, cases, st, pc)
+    o = setup_open_box(new_oracle(), I, cases, I.SOLVER_BiCGSTAB, I.PRECOND_AMG)
+    g.iterate(5)
+    o.iterate(5)
+    compare(g, o, 1e-8)
+    assert g.last_norms()[3] <= 1e-12
