@@ -49,6 +49,13 @@ template <typename T>
 void launch_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
                    const T *r, T *z);
 
+// BiCGSTAB direction update fused with the Jacobi preconditioner:
+// which 0: p = r + beta (p - omega v), ph = D^-1 p; which 1: p = r - alpha v
+// (the s vector), ph = D^-1 p. Results equal launch_bicg_p / _s + launch_jacobi.
+template <typename T>
+void launch_bicg_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G,
+                        const uint8_t *flag, const KScal *ks, int which, const T *r, const T *v, T *p, T *ph);
+
 // one colour of red-black Gauss-Seidel on the fluid rows (cells with
 // (i + j + k) % 2 == color)
 template <typename T>

@@ -381,16 +381,70 @@ void launch_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const G
 // ---------------------------------------------------------------------------
 // Jacobi preconditioner z = D^-1 r
 // ---------------------------------------------------------------------------
+// Two cells of a row per lane. Fluid rows: r / diag; solid and pinned rows:
+// r (identity); ghost rows: written afterwards by k_jacobi_ghost.
+template <typename T>
+__device__ inline T jacobi_row(const Dims &d, const SysCoef<T> &S, unsigned f, int i, int j, int k, long q, T rv) {
+    const T diag = stencil_diag(S, min(i, d.nx - 1), j, k, q);
+    if (q == S.pin || f == F_SOLID) return rv;
+    return f == F_FLUID && diag != T(0) ? rv / diag : T(0);
+}
+
+template <typename T>
+__device__ inline void store_pair(const Dims &d, int i0, long q0, T *a, T v0, T v1) {
+    using T2 = typename Vec2<T>::type;
+    if (i0 + 1 < d.nx) {
+        T2 o;
+        o.x = v0;
+        o.y = v1;
+        *reinterpret_cast<T2 *>(a + q0) = o;
+    } else {
+        a[q0] = v0;
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_jacobi(Dims d, SysCoef<T> S, const uint8_t *flag, const T *r, T *z) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-    if (i >= d.nx || j >= d.ny) return;
-    const long q = cidx(d, i, j, k);
-    const uint8_t f = flag[q];
-    if (q == S.pin || f == F_SOLID) { z[q] = r[q]; return; }
-    if (f != F_FLUID) return;
-    const T diag = stencil_diag(S, i, j, k, q);
-    z[q] = diag != T(0) ? r[q] / diag : T(0);
+    using T2 = typename Vec2<T>::type;
+    const uint3 bk = xcd_block(1);
+    const int i0 = 2 * (bk.x * 64 + threadIdx.x), j = bk.y * 4 + threadIdx.y, k = bk.z;
+    if (i0 >= d.nx || j >= d.ny) return;
+    const long q0 = cidx(d, i0, j, k);
+    const unsigned fv = *reinterpret_cast<const unsigned short *>(flag + q0);
+    const T2 rv = *reinterpret_cast<const T2 *>(r + q0);
+    store_pair(d, i0, q0, z, jacobi_row(d, S, fv & 0xFFu, i0, j, k, q0, rv.x),
+               jacobi_row(d, S, fv >> 8, i0 + 1, j, k, q0 + 1, rv.y));
+}
+
+// BiCGSTAB direction updates fused with the Jacobi preconditioner:
+//   SJ = 0: p = r + beta (p - omega v), ph = D^-1 p
+//   SJ = 1: s = r - alpha v,            sh = D^-1 s
+// Same arithmetic as k_bicg_p / k_bicg_s followed by k_jacobi.
+template <typename T, int SJ>
+__global__ void __launch_bounds__(256) k_bicg_jacobi(Dims d, SysCoef<T> S, const uint8_t *flag, const KScal *ks,
+                                                     const T *r, const T *v, T *p, T *ph) {
+    using T2 = typename Vec2<T>::type;
+    const uint3 bk = xcd_block(1);
+    const int i0 = 2 * (bk.x * 64 + threadIdx.x), j = bk.y * 4 + threadIdx.y, k = bk.z;
+    if (i0 >= d.nx || j >= d.ny) return;
+    const long q0 = cidx(d, i0, j, k);
+    const unsigned fv = *reinterpret_cast<const unsigned short *>(flag + q0);
+    const T2 rv = *reinterpret_cast<const T2 *>(r + q0);
+    const T2 vv = *reinterpret_cast<const T2 *>(v + q0);
+    T n0, n1;
+    if (SJ == 0) {
+        const T2 pv = *reinterpret_cast<const T2 *>(p + q0);
+        const T beta = (T)ks->beta, omega = (T)ks->omega;
+        n0 = rv.x + beta * (pv.x - omega * vv.x);
+        n1 = rv.y + beta * (pv.y - omega * vv.y);
+    } else {
+        const T alpha = (T)ks->alpha;
+        n0 = rv.x - alpha * vv.x;
+        n1 = rv.y - alpha * vv.y;
+    }
+    store_pair(d, i0, q0, p, n0, n1);
+    store_pair(d, i0, q0, ph, jacobi_row(d, S, fv & 0xFFu, i0, j, k, q0, n0),
+               jacobi_row(d, S, fv >> 8, i0 + 1, j, k, q0 + 1, n1));
 }
 
 template <typename T>
@@ -402,12 +456,26 @@ __global__ void k_jacobi_ghost(GhostRows<T> G, int is_p, const T *r, T *z) {
     z[c] = dg != T(0) ? r[c] / dg : r[c];
 }
 
+// one cell pair per lane, one plane per block
+static inline dim3 jgrid(const Dims &d) { return dim3(((d.nx + 1) / 2 + 63) / 64, (d.ny + 3) / 4, d.nz); }
+
 template <typename T>
 void launch_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
                    const T *r, T *z) {
-    hipLaunchKernelGGL((k_jacobi<T>), igrid(d), dim3(64, 4), 0, st, d, S, flag, r, z);
+    hipLaunchKernelGGL((k_jacobi<T>), jgrid(d), dim3(64, 4), 0, st, d, S, flag, r, z);
     const int gb = (G.ng + 255) / 256;
     if (gb) hipLaunchKernelGGL((k_jacobi_ghost<T>), dim3(gb), dim3(256), 0, st, G, S.is_p, r, z);
+}
+
+template <typename T>
+void launch_bicg_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G,
+                        const uint8_t *flag, const KScal *ks, int which, const T *r, const T *v, T *p, T *ph) {
+    if (which == 0)
+        hipLaunchKernelGGL((k_bicg_jacobi<T, 0>), jgrid(d), dim3(64, 4), 0, st, d, S, flag, ks, r, v, p, ph);
+    else
+        hipLaunchKernelGGL((k_bicg_jacobi<T, 1>), jgrid(d), dim3(64, 4), 0, st, d, S, flag, ks, r, v, p, ph);
+    const int gb = (G.ng + 255) / 256;
+    if (gb) hipLaunchKernelGGL((k_jacobi_ghost<T>), dim3(gb), dim3(256), 0, st, G, S.is_p, p, ph);
 }
 
 // ---------------------------------------------------------------------------
@@ -1294,6 +1362,8 @@ void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, 
     template void launch_bicg_init<T>(hipStream_t, const Dims &, T *, T *, T *, T *, T *, const T *);              \
     template void launch_bicg_p<T>(hipStream_t, const Dims &, const KScal *, const T *, const T *, T *);           \
     template void launch_bicg_s<T>(hipStream_t, const Dims &, const KScal *, const T *, const T *, T *);           \
+    template void launch_bicg_jacobi<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,       \
+                                        const uint8_t *, const KScal *, int, const T *, const T *, T *, T *);      \
     template void launch_bicg_xr<T>(hipStream_t, const Dims &, const KScal *, const T *, const T *, const T *,     \
                                     const T *, const T *, T *, T *, const NormSpec &, double *);                   \
     template void launch_zero<T>(hipStream_t, const Dims &, T *);                                                  \

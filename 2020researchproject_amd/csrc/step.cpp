@@ -39,7 +39,12 @@ static hipEvent_t get_event(IBMSolver *s) {
     }
     hipEvent_t e;
     HipGlobalLock lk(hip_global_mutex());
-    HIP_CHECK(hipEventCreate(&e));
+    // timing-only events: no system-scope fence when they complete, so the
+    // interval around a kernel does not include a cache write-back
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
+        (void)hipGetLastError();
+        HIP_CHECK(hipEventCreate(&e));
+    }
     return e;
 }
 
@@ -707,13 +712,21 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
     while (it < maxit && rn > tol * bn) {
         it++;
         snprintf(nm, sizeof nm, "bicg_vec");
-        TIMED("bicg_vec", 4 * cb, launch_bicg_p<T>(s->stream, d, s->ks.p, r, v, p));
-        precond(p, ph);
+        if (prec == 1) {
+            TIMED("bicg_jacobi", 5 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 0, r, v, p, ph));
+        } else {
+            TIMED("bicg_vec", 4 * cb, launch_bicg_p<T>(s->stream, d, s->ks.p, r, v, p));
+            precond(p, ph);
+        }
         snprintf(nm, sizeof nm, "spmv_%s", tag);
         TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, ph, v, 1, rh, s->partial.p));
         launch_finalize(s->stream, 0, s->partial.p, na, 1, s->ks.p);
-        TIMED("bicg_vec", 3 * cb, launch_bicg_s<T>(s->stream, d, s->ks.p, r, v, sv));
-        precond(sv, sh);
+        if (prec == 1) {
+            TIMED("bicg_jacobi", 4 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 1, r, v, sv, sh));
+        } else {
+            TIMED("bicg_vec", 3 * cb, launch_bicg_s<T>(s->stream, d, s->ks.p, r, v, sv));
+            precond(sv, sh);
+        }
         TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, sh, t, 2, sv, s->partial.p));
         launch_finalize(s->stream, 1, s->partial.p, na, 2, s->ks.p);
         TIMED("bicg_xr", 8 * cb, launch_bicg_xr<T>(s->stream, d, s->ks.p, ph, sh, sv, t, rh, x, r, ns, s->partial.p));

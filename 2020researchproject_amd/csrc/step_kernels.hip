@@ -90,52 +90,52 @@ __global__ void __launch_bounds__(256) k_conv_rhs(Geo g, StepParams sp, const ui
     const int k = blockIdx.z;
     double nrm[3] = {0, 0, 0};
     if (i < d.nx && j < d.ny) {
+        // every cell is computed (the neighbours of interior cells are in the
+        // padded range) and non-fluid results are replaced at the end, so all
+        // loads issue before the first wait
         const long q = cidx(d, i, j, k);
         const long sy = d.sy, sz = d.sz;
-        if (flag[q] != F_FLUID) {
-            N1[q] = N2[q] = N3[q] = nan_d();
-            b1[q] = b2[q] = b3[q] = 0;
-        } else {
-            const double dxm = C1(g.dx, i - 1), dx0 = C1(g.dx, i), dxp = C1(g.dx, i + 1);
-            const double dym = C1(g.dy, j - 1), dy0 = C1(g.dy, j), dyp = C1(g.dy, j + 1);
-            const double dzm = C1(g.dz, k - 1), dz0 = C1(g.dz, k), dzp = C1(g.dz, k + 1);
-            const double Uw = U1[q], Ue = U1[q + 1], Vs = U2[q], Vn = U2[q + sy], Wd = U3[q], Wu = U3[q + sz];
-            const double kW = C1(g.kW, i), kE = C1(g.kE, i), kS = C1(g.kS, j), kN = C1(g.kN, j);
-            const double kD = C1(g.kD, k), kU = C1(g.kU, k);
-            const double kc = 1 - kW - kE - kS - kN - kD - kU;
-            const double ddx = C1(g.xc, i + 1) - C1(g.xc, i - 1);
-            const double ddy = C1(g.yc, j + 1) - C1(g.yc, j - 1);
-            const double ddz = C1(g.zc, k + 1) - C1(g.zc, k - 1);
-            const double *uu[3] = {u1, u2, u3};
-            const double *Np[3] = {Np1, Np2, Np3};
-            double *NN[3] = {N1, N2, N3};
-            double *bb[3] = {b1, b2, b3};
-            const double dp[3] = {(p[q + 1] - p[q - 1]) / ddx, (p[q + sy] - p[q - sy]) / ddy,
-                                  (p[q + sz] - p[q - sz]) / ddz};
+        const bool fl = flag[q] == F_FLUID;
+        const double dxm = C1(g.dx, i - 1), dx0 = C1(g.dx, i), dxp = C1(g.dx, i + 1);
+        const double dym = C1(g.dy, j - 1), dy0 = C1(g.dy, j), dyp = C1(g.dy, j + 1);
+        const double dzm = C1(g.dz, k - 1), dz0 = C1(g.dz, k), dzp = C1(g.dz, k + 1);
+        const double Uw = U1[q], Ue = U1[q + 1], Vs = U2[q], Vn = U2[q + sy], Wd = U3[q], Wu = U3[q + sz];
+        const double kW = C1(g.kW, i), kE = C1(g.kE, i), kS = C1(g.kS, j), kN = C1(g.kN, j);
+        const double kD = C1(g.kD, k), kU = C1(g.kU, k);
+        const double kc = 1 - kW - kE - kS - kN - kD - kU;
+        const double ddx = C1(g.xc, i + 1) - C1(g.xc, i - 1);
+        const double ddy = C1(g.yc, j + 1) - C1(g.yc, j - 1);
+        const double ddz = C1(g.zc, k + 1) - C1(g.zc, k - 1);
+        const double *uu[3] = {u1, u2, u3};
+        const double *Np[3] = {Np1, Np2, Np3};
+        double *NN[3] = {N1, N2, N3};
+        double *bb[3] = {b1, b2, b3};
+        const double dp[3] = {(p[q + 1] - p[q - 1]) / ddx, (p[q + sy] - p[q - sy]) / ddy,
+                              (p[q + sz] - p[q - sz]) / ddz};
 #pragma unroll
-            for (int l = 0; l < 3; l++) {
-                const double *a = uu[l];
-                const double c = a[q], xm = a[q - 1], xp = a[q + 1], ym = a[q - sy], yp = a[q + sy];
-                const double zm = a[q - sz], zp = a[q + sz];
-                const double uw = (xm * dx0 + c * dxm) / (dxm + dx0);
-                const double ue = (c * dxp + xp * dx0) / (dx0 + dxp);
-                const double us = (ym * dy0 + c * dym) / (dym + dy0);
-                const double un = (c * dyp + yp * dy0) / (dy0 + dyp);
-                const double ud = (zm * dz0 + c * dzm) / (dzm + dz0);
-                const double uu_ = (c * dzp + zp * dz0) / (dz0 + dzp);
-                const double n = (Ue * ue - Uw * uw) / dx0 + (Vn * un - Vs * us) / dy0 + (Wu * uu_ - Wd * ud) / dz0;
-                NN[l][q] = n;
-                double divt = 0;
-                if (tau.on) {
-                    const double *t1 = tau.t[tau.slot[l][0]], *t2 = tau.t[tau.slot[l][1]], *t3 = tau.t[tau.slot[l][2]];
-                    divt = (t1[q + 1] - t1[q - 1]) / ddx + (t2[q + sy] - t2[q - sy]) / ddy + (t3[q + sz] - t3[q - sz]) / ddz;
-                }
-                const double b = -sp.dt / 2 * (3 * n - Np[l][q]) - sp.dt * dp[l] + kc * c
-                    + kW * xm + kE * xp + kS * ym + kN * yp + kD * zm + kU * zp
-                    + sp.dt * (sp.fa[l] ? sp.fa[l][q] : sp.force[l]) - sp.dt * divt;
-                bb[l][q] = b;
-                nrm[l] = b * b;
+        for (int l = 0; l < 3; l++) {
+            const double *a = uu[l];
+            const double c = a[q], xm = a[q - 1], xp = a[q + 1], ym = a[q - sy], yp = a[q + sy];
+            const double zm = a[q - sz], zp = a[q + sz];
+            const double npl = Np[l][q];
+            const double uw = (xm * dx0 + c * dxm) / (dxm + dx0);
+            const double ue = (c * dxp + xp * dx0) / (dx0 + dxp);
+            const double us = (ym * dy0 + c * dym) / (dym + dy0);
+            const double un = (c * dyp + yp * dy0) / (dy0 + dyp);
+            const double ud = (zm * dz0 + c * dzm) / (dzm + dz0);
+            const double uu_ = (c * dzp + zp * dz0) / (dz0 + dzp);
+            const double n = (Ue * ue - Uw * uw) / dx0 + (Vn * un - Vs * us) / dy0 + (Wu * uu_ - Wd * ud) / dz0;
+            double divt = 0;
+            if (tau.on) {
+                const double *t1 = tau.t[tau.slot[l][0]], *t2 = tau.t[tau.slot[l][1]], *t3 = tau.t[tau.slot[l][2]];
+                divt = (t1[q + 1] - t1[q - 1]) / ddx + (t2[q + sy] - t2[q - sy]) / ddy + (t3[q + sz] - t3[q - sz]) / ddz;
             }
+            const double b = -sp.dt / 2 * (3 * n - npl) - sp.dt * dp[l] + kc * c
+                + kW * xm + kE * xp + kS * ym + kN * yp + kD * zm + kU * zp
+                + sp.dt * (sp.fa[l] ? sp.fa[l][q] : sp.force[l]) - sp.dt * divt;
+            NN[l][q] = fl ? n : nan_d();
+            bb[l][q] = fl ? b : 0.0;
+            nrm[l] = fl ? b * b : 0.0;
         }
     }
     block_sum_store<3>(nrm, partial, block_linear());
