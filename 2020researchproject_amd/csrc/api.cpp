@@ -53,6 +53,9 @@ IBMSolver *IBMSolver_new(const int num_process, const int rank, const int Px, co
     DevBytesScope account(&s->dev_bytes);
     HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_CHECK(hipHostMalloc((void **)&s->ks_h, sizeof(KScal), hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc((void **)&s->pub_h, sizeof(IBMSolver::KPub), hipHostMallocCoherent | hipHostMallocMapped));
+    memset((void *)s->pub_h, 0, sizeof(IBMSolver::KPub));
+    HIP_CHECK(hipHostGetDevicePointer((void **)&s->pub_d, s->pub_h, 0));
     HIP_CHECK(hipHostMalloc((void **)&s->sums_h, kSumSlots * sizeof(double), hipHostMallocDefault));
     HIP_CHECK(hipHostMalloc((void **)&s->gm_h, 64 * sizeof(double), hipHostMallocDefault));
     s->ks.alloc(1);
@@ -69,6 +72,7 @@ void IBMSolver_destroy(IBMSolver *s) {
     for (auto e : s->event_pool) (void)hipEventDestroy(e);
     for (auto &g : s->vgraphs) (void)hipGraphExecDestroy(g.exec);
     (void)hipHostFree(s->ks_h);
+    (void)hipHostFree(s->pub_h);
     (void)hipHostFree(s->sums_h);
     (void)hipHostFree(s->gm_h);
     (void)hipStreamDestroy(s->stream);

@@ -224,6 +224,16 @@ struct _ibm_solver {
     ibm::DBuf<double> partial, sums;  // sums: kSumSlots doubles
     ibm::DBuf<ibm::KScal> ks;
     ibm::KScal *ks_h = nullptr;      // pinned
+    // Krylov scalars published by the finalize kernel into device-mapped,
+    // coherent host memory with an epoch counter (the host waits on it)
+    struct KPub {
+        ibm::KScal ks;
+        unsigned epoch;
+        unsigned pad;
+    };
+    KPub *pub_h = nullptr;           // host address
+    KPub *pub_d = nullptr;           // device address of the same memory
+    unsigned pub_epoch = 0;
     double *sums_h = nullptr;        // pinned
     double *gm_h = nullptr;          // pinned, 64 scalars of the GMRES / PCG solvers
 

@@ -120,8 +120,10 @@ void launch_lincomb(hipStream_t st, const Dims &d, const T *z, double f, T *y, i
 
 // finalize ops: 0 = alpha from dot(rh, v); 1 = omega from (t.s, t.t);
 // 2 = rho/beta/norms after the x, r update; 3 = store b norms and rho1 at
-// start; 4 = store residual norms only
-void launch_finalize(hipStream_t st, int op, const double *partial, long nblk, int nv, KScal *ks);
+// start; 4 = store residual norms only. pub (device-mapped host memory, or
+// null): the scalars are also copied there, then *pub_epoch = epoch.
+void launch_finalize(hipStream_t st, int op, const double *partial, long nblk, int nv, KScal *ks,
+                     KScal *pub = nullptr, unsigned *pub_epoch = nullptr, unsigned epoch = 0);
 // out[a] = sum (or max) over blocks of partial[b * nv + a]
 void launch_sum_partials(hipStream_t st, const double *partial, long nblk, int nv, double *out, int reduce_max);
 

@@ -797,7 +797,8 @@ __device__ inline void sum_columns(const double *partial, long nblk, int nv, int
 }
 
 __global__ void __launch_bounds__(kFinThreads) k_finalize(int op, const double *partial, long nblk, int nv,
-                                                          KScal *ks) {
+                                                          KScal *ks, KScal *pub, unsigned *pub_epoch,
+                                                          unsigned epoch) {
     __shared__ double sh[4][kFinThreads];
     const int t = threadIdx.x;
     sum_columns(partial, nblk, nv, 0, sh);
@@ -834,6 +835,13 @@ __global__ void __launch_bounds__(kFinThreads) k_finalize(int op, const double *
         break;
     default:;
     }
+    if (pub) {
+        // host copy of the scalars, then the epoch that tells the host they
+        // are complete (system-scope fence between the two)
+        *pub = *ks;
+        __threadfence_system();
+        *reinterpret_cast<volatile unsigned *>(pub_epoch) = epoch;
+    }
 }
 
 // sums nv partial columns into out[0..nv) (same fixed order as k_finalize)
@@ -849,8 +857,10 @@ void launch_sum_partials(hipStream_t st, const double *partial, long nblk, int n
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kFinThreads), 0, st, partial, nblk, nv, out, reduce_max);
 }
 
-void launch_finalize(hipStream_t st, int op, const double *partial, long nblk, int nv, KScal *ks) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, st, op, partial, nblk, nv, ks);
+void launch_finalize(hipStream_t st, int op, const double *partial, long nblk, int nv, KScal *ks, KScal *pub,
+                     unsigned *pub_epoch, unsigned epoch) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, st, op, partial, nblk, nv, ks, pub, pub_epoch,
+                       epoch);
 }
 
 // ---------------------------------------------------------------------------

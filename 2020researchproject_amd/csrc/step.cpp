@@ -189,9 +189,30 @@ static void read_sums(IBMSolver *s, int nv) {
     HIP_CHECK(hipStreamSynchronize(s->stream));
 }
 
-static void read_ks(IBMSolver *s) {
-    HIP_CHECK(hipMemcpyAsync(s->ks_h, s->ks.p, sizeof(KScal), hipMemcpyDeviceToHost, s->stream));
-    HIP_CHECK(hipStreamSynchronize(s->stream));
+// Finalize kernel `op` over the block partials, with the resulting scalars
+// published into host memory, then a wait for them there: no copy and no
+// stream synchronisation on the Krylov loop's path (each end of iteration
+// otherwise costs a copy and a wake-up of the host thread). The stream is
+// polled now and then so that an error or a lost publication cannot make
+// the wait endless.
+static void finalize_read(IBMSolver *s, int op, long nblk, int nv) {
+    const unsigned e = ++s->pub_epoch;
+    launch_finalize(s->stream, op, s->partial.p, nblk, nv, s->ks.p, &s->pub_d->ks, &s->pub_d->epoch, e);
+    unsigned *ep = &s->pub_h->epoch;
+    for (long n = 1; __atomic_load_n(ep, __ATOMIC_ACQUIRE) != e; n++) {
+        if ((n & 1023) == 0) {
+            const hipError_t q = hipStreamQuery(s->stream);
+            if (q == hipSuccess) {
+                if (__atomic_load_n(ep, __ATOMIC_ACQUIRE) == e) break;
+                fprintf(stderr, "error: Krylov scalars were not published (epoch %u, expected %u)\n",
+                        __atomic_load_n(ep, __ATOMIC_ACQUIRE), e);
+                abort();
+            }
+            if (q != hipErrorNotReady) HIP_CHECK(q);
+        }
+        __builtin_ia32_pause();
+    }
+    memcpy((void *)s->ks_h, (const void *)&s->pub_h->ks, sizeof(KScal));
 }
 
 // ---------------------------------------------------------------------------
@@ -690,8 +711,7 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
     launch_bicg_init<T>(s->stream, d, r, rh, p, v, x, b);
     // norms of b (= r0) through the residual kernel with x = 0
     TIMED("residual", 3 * cb, launch_residual<T>(s->stream, d, S, G, flag, b, x, r, ns, s->partial.p));
-    launch_finalize(s->stream, 3, s->partial.p, na, 2, s->ks.p);
-    read_ks(s);
+    finalize_read(s, 3, na, 2);
     const double bn = bnorm > 0 ? bnorm : std::sqrt(ns.on ? s->ks_h->bbn : s->ks_h->bb);
     if (bn == 0) { *final_rel = 0; return 0; }
     double rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
@@ -730,8 +750,7 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, sh, t, 2, sv, s->partial.p));
         launch_finalize(s->stream, 1, s->partial.p, na, 2, s->ks.p);
         TIMED("bicg_xr", 8 * cb, launch_bicg_xr<T>(s->stream, d, s->ks.p, ph, sh, sv, t, rh, x, r, ns, s->partial.p));
-        launch_finalize(s->stream, 2, s->partial.p, nv, 3, s->ks.p);
-        read_ks(s);
+        finalize_read(s, 2, nv, 3);
         rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
         const bool bad = s->ks_h->breakdown || !std::isfinite(rn);
         if (restart > 0 && it < maxit && (bad || it % restart == 0) && !(rn <= tol * bn)) {
@@ -741,8 +760,7 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
             // true residual; restarting resets them, keeping x.
             TIMED("residual", 3 * cb, launch_residual<T>(s->stream, d, S, G, flag, b, x, r, ns, s->partial.p));
             launch_bicg_restart<T>(s->stream, d, r, rh, p, v);
-            launch_finalize(s->stream, 3, s->partial.p, na, 2, s->ks.p);
-            read_ks(s);
+            finalize_read(s, 3, na, 2);
             rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
             if (!std::isfinite(rn)) break;
             continue;
@@ -751,15 +769,13 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
     }
     // true residual of the final iterate
     TIMED("residual", 3 * cb, launch_residual<T>(s->stream, d, S, G, flag, b, x, r, ns, s->partial.p));
-    launch_finalize(s->stream, 4, s->partial.p, na, 2, s->ks.p);
-    read_ks(s);
+    finalize_read(s, 4, na, 2);
     const double rt = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
     if (restart > 0 && it < maxit && std::isfinite(rt) && rt > tol * bn) {
         // the recurrence met the tolerance but the true residual did not:
         // restart from it (r already holds b - A x)
         launch_bicg_restart<T>(s->stream, d, r, rh, p, v);
-        launch_finalize(s->stream, 3, s->partial.p, na, 2, s->ks.p);
-        read_ks(s);
+        finalize_read(s, 3, na, 2);
         rn = rt;
         continue;
     }
@@ -791,8 +807,7 @@ static double true_residual(IBMSolver *s, const double *b, const double *x, doub
     const Dims &d = s->d;
     TIMED("residual", 3 * ncells(s) * 8,
           launch_residual<double>(s->stream, d, s->sys[3].S64, s->gp.G64, s->flag.p, b, x, r, ns, s->partial.p));
-    launch_finalize(s->stream, 4, s->partial.p, apply_partials(d, s->gp.G64.ng), 2, s->ks.p);
-    read_ks(s);
+    finalize_read(s, 4, apply_partials(d, s->gp.G64.ng), 2);
     return std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
 }
 
@@ -964,8 +979,7 @@ static int refine_mixed(IBMSolver *s, int sk, const double *b, double *x, int pr
     const double cb = ncells(s);
     launch_zero<double>(s->stream, d, x);
     TIMED("residual64", 3 * cb * 8, launch_residual<double>(s->stream, d, S64, GD.G64, flag, b, x, r64, ns, s->partial.p));
-    launch_finalize(s->stream, 4, s->partial.p, na, 2, s->ks.p);
-    read_ks(s);
+    finalize_read(s, 4, na, 2);
     const double bn = bnorm > 0 ? bnorm : std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
     if (bn == 0) { *final_rel = 0; return 0; }
     double rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
@@ -982,8 +996,7 @@ static int refine_mixed(IBMSolver *s, int sk, const double *b, double *x, int pr
         TIMED("add_f2d", cb * 20, launch_add_f2d(s->stream, d, e32, x));
         TIMED("residual64", 3 * cb * 8,
               launch_residual<double>(s->stream, d, S64, GD.G64, flag, b, x, r64, ns, s->partial.p));
-        launch_finalize(s->stream, 4, s->partial.p, na, 2, s->ks.p);
-        read_ks(s);
+        finalize_read(s, 4, na, 2);
         const double rn_new = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
         if (!std::isfinite(rn_new)) { rn = rn_new; break; }
         rn = rn_new;
