@@ -56,6 +56,18 @@ template <typename T>
 void launch_bicg_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G,
                         const uint8_t *flag, const KScal *ks, int which, const T *r, const T *v, T *p, T *ph);
 
+// BiCGSTAB direction update in FP64 (which 0: p = r + beta (p - omega v);
+// 1: p = r - alpha v, the s vector) that also writes b32 = (float) p and,
+// when x32 is not null, the first smoother colour on b32 with system *S
+// (the values launch_d2f_first writes).
+void launch_bicg_d2f(hipStream_t st, const Dims &d, const SysCoef<float> *S, const uint8_t *flag, const KScal *ks,
+                     int which, const double *r, const double *v, double *p, float *b32, float *x32);
+// y = A x for x stored in FP32 (x32): as launch_apply (modes 1 and 2) on x =
+// (double) x32, which is also written to x.
+void launch_apply_f32in(hipStream_t st, const Dims &d, const SysCoef<double> &S, const GhostRows<double> &G,
+                        const uint8_t *flag, const float *x32, double *x, double *y, int mode, const double *w,
+                        double *partial);
+
 // one colour of red-black Gauss-Seidel on the fluid rows (cells with
 // (i + j + k) % 2 == color)
 template <typename T>
@@ -129,6 +141,9 @@ void launch_sum_partials(hipStream_t st, const double *partial, long nblk, int n
 
 // precision conversion / accumulation over interior cells
 void launch_d2f(hipStream_t st, const Dims &d, const double *a, float *b);
+// b = (float) a and x = the first smoother colour from a zero start on b
+void launch_d2f_first(hipStream_t st, const Dims &d, const SysCoef<float> &S, const uint8_t *flag, const double *a,
+                      float *b, float *x);
 void launch_add_f2d(hipStream_t st, const Dims &d, const float *e, double *x);
 void launch_f2d(hipStream_t st, const Dims &d, const float *e, double *x);
 template <typename T>
@@ -152,17 +167,23 @@ struct MGTransfer {
 template <typename T>
 bool launch_mg_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
                         const T *x, T *r);
-// bc = restriction of the fluid rows' residual b - A x (one pass, no r)
+// bc = restriction of the fluid rows' residual b - A x (one pass, no r).
+// xc (with the coarse system Sc and flags flag_c), when not null, receives
+// the first colour of the coarse smoother from a zero start (the values
+// launch_rbgs_color(..., color 0, zero_in 1) would write).
 template <typename T>
 void launch_resid_restrict(hipStream_t st, const Dims &df, const SysCoef<T> &S, const uint8_t *flag_f, const T *b,
                            const T *x, const Dims &dc, const MGTransfer &tr, const double *dxyz_f,
-                           const double *dxyz_c, T *bc);
-// rf_zero_off_fluid: rf is 0 at non-fluid cells, so no flags are read
+                           const double *dxyz_c, T *bc, const SysCoef<T> *Sc = nullptr,
+                           const uint8_t *flag_c = nullptr, T *xc = nullptr);
+// rf_zero_off_fluid: rf is 0 at non-fluid cells, so no flags are read;
+// Sc, flag_c, xc as for launch_resid_restrict
 template <typename T>
 void launch_restrict(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
                      const T *rf, const double *dxf, const double *dyf, const double *dzf, const double *dxc,
                      const double *dyc, const double *dzc, T *bc,
-                     bool rf_zero_off_fluid);
+                     bool rf_zero_off_fluid, const SysCoef<T> *Sc = nullptr, const uint8_t *flag_c = nullptr,
+                     T *xc = nullptr);
 template <typename T>
 void launch_prolong(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
                     const uint8_t *flag_c, const T *ec, T *xf);

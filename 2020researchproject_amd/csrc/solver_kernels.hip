@@ -97,23 +97,33 @@ struct Vec2<double> {
 // neighbours as 2-vectors (x row, the rows j -/+ 1 and the planes k -/+ 1),
 // one cell-sized load on each side of the pair, flags and face masks as one
 // 16-bit load. Writes the rows' diagonal-weighted terms: yv[o] = diag x -
-// off, xq[o] = x, for o = 0, 1. Same arithmetic per cell as stencil().
-template <typename T>
-__device__ inline void pair_apply(const Dims &d, const SysCoef<T> &S, const T *x, int i0, int j, int k, long q0,
+// off, xq[o] = x, for o = 0, 1. Same arithmetic per cell as stencil(). x may
+// be stored in a narrower type TX (values converted exactly on load).
+template <typename T, typename TX>
+__device__ inline typename Vec2<T>::type load_pair(const TX *p) {
+    const typename Vec2<TX>::type v = *reinterpret_cast<const typename Vec2<TX>::type *>(p);
+    typename Vec2<T>::type o;
+    o.x = (T)v.x;
+    o.y = (T)v.y;
+    return o;
+}
+
+template <typename T, typename TX = T>
+__device__ inline void pair_apply(const Dims &d, const SysCoef<T> &S, const TX *x, int i0, int j, int k, long q0,
                                   T yv[2], T xq[2]) {
     using T2 = typename Vec2<T>::type;
     const AxisCoef<T> &AX = S.ax[0], &AY = S.ax[1], &AZ = S.ax[2];
     const int jm = (j == 0 && AY.periodic) ? AY.n - 1 : j - 1, jp = (j == AY.n - 1 && AY.periodic) ? 0 : j + 1;
     const int km = (k == 0 && AZ.periodic) ? AZ.n - 1 : k - 1, kp = (k == AZ.n - 1 && AZ.periodic) ? 0 : k + 1;
-    const T2 xc = *reinterpret_cast<const T2 *>(x + q0);
-    const T2 ym = *reinterpret_cast<const T2 *>(x + q0 + (long)(jm - j) * d.sy);
-    const T2 yp = *reinterpret_cast<const T2 *>(x + q0 + (long)(jp - j) * d.sy);
-    const T2 zm = *reinterpret_cast<const T2 *>(x + q0 + (long)(km - k) * d.sz);
-    const T2 zp = *reinterpret_cast<const T2 *>(x + q0 + (long)(kp - k) * d.sz);
-    T xl = x[q0 - 1], xr = x[q0 + 2];
+    const T2 xc = load_pair<T>(x + q0);
+    const T2 ym = load_pair<T>(x + q0 + (long)(jm - j) * d.sy);
+    const T2 yp = load_pair<T>(x + q0 + (long)(jp - j) * d.sy);
+    const T2 zm = load_pair<T>(x + q0 + (long)(km - k) * d.sz);
+    const T2 zp = load_pair<T>(x + q0 + (long)(kp - k) * d.sz);
+    T xl = (T)x[q0 - 1], xr = (T)x[q0 + 2];
     if (AX.periodic) {
-        if (i0 == 0) xl = x[q0 + AX.n - 1];
-        if (i0 + 1 == AX.n - 1) xr = x[q0 - i0];
+        if (i0 == 0) xl = (T)x[q0 + AX.n - 1];
+        if (i0 + 1 == AX.n - 1) xr = (T)x[q0 - i0];
     }
     const unsigned mm = S.skip ? *reinterpret_cast<const unsigned short *>(S.fmask + q0) : 0u;
     using P4 = typename Packed4<T>::type;
@@ -126,7 +136,7 @@ __device__ inline void pair_apply(const Dims &d, const SysCoef<T> &S, const T *x
         const unsigned m = (mm >> (8 * o)) & 0xFFu;
         // periodic x, cell i0 + 1 the last of its row: the wrap neighbour
         T xp = o ? xr : xc.y;
-        if (o == 0 && AX.periodic && i0 == AX.n - 1) xp = x[q0 - i0];
+        if (o == 0 && AX.periodic && i0 == AX.n - 1) xp = (T)x[q0 - i0];
         const T xm = o ? xc.x : xl;
         const P4 cx = o ? cx1 : cx0;
         T diag = S.c0, off = 0;
@@ -179,6 +189,19 @@ __device__ inline T stencil_diag(const SysCoef<T> &S, int i, int j, int k, long 
     return diag;
 }
 
+// First colour of a red-black sweep from a zero start (colour 0: cells with
+// i + j + k even), the value the one-colour smoother writes at (i, j, k):
+// same arithmetic (the neighbour sum is 0), so a kernel that produces a
+// level's right-hand side can also write this colour and save that launch.
+template <typename T>
+__device__ inline T first_colour(const SysCoef<T> &S, unsigned f, int i, int j, int k, long q, T bv) {
+    if ((i + j + k) & 1) return T(0);
+    if (q == S.pin || f == F_SOLID) return bv;
+    if (f != F_FLUID) return T(0);
+    const T diag = stencil_diag(S, i, j, k, q);
+    return diag != T(0) ? (bv + T(0)) / diag : T(0);
+}
+
 // ---------------------------------------------------------------------------
 // y = A x (interior non-ghost rows); mode 1 adds dot(w, y), mode 2 adds
 // dot(y, w) and dot(y, y).
@@ -186,10 +209,12 @@ __device__ inline T stencil_diag(const SysCoef<T> &S, int i, int j, int k, long 
 // The kernels below compute the stencil of every cell before looking at its
 // flag (solid and ghost cells have in-bounds neighbours too; their results
 // are discarded), so all loads of a thread issue before the first wait.
-template <typename T, int MODE>
+// TX: storage type of x; xo, when not null, receives x in type T (the
+// operand of a later vector update).
+template <typename T, int MODE, typename TX = T>
 __global__ void __launch_bounds__(256) k_apply(Dims d, SysCoef<T> S, const uint8_t *__restrict__ flag,
-                                               const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ w,
-                                               double *partial) {
+                                               const TX *__restrict__ x, T *__restrict__ y, const T *__restrict__ w,
+                                               double *partial, T *__restrict__ xo) {
     using T2 = typename Vec2<T>::type;
     const int i0 = 2 * (blockIdx.x * 64 + threadIdx.x), j = blockIdx.y * 4 + threadIdx.y;
     const int k0 = blockIdx.z * kRedZP, k1 = min(k0 + kRedZP, d.nz);
@@ -225,8 +250,15 @@ __global__ void __launch_bounds__(256) k_apply(Dims d, SysCoef<T> S, const uint8
                 yy.x = v[0];
                 yy.y = v[1];
                 *reinterpret_cast<T2 *>(y + q0) = yy;
+                if (xo) {
+                    T2 xx;
+                    xx.x = xq[kk][0];
+                    xx.y = xq[kk][1];
+                    *reinterpret_cast<T2 *>(xo + q0) = xx;
+                }
             } else {
                 y[q0] = v[0];
+                if (xo) xo[q0] = xq[kk][0];
             }
 #pragma unroll
             for (int o = 0; o < 2; o++) {
@@ -270,14 +302,30 @@ void launch_apply(hipStream_t st, const Dims &d, const SysCoef<T> &S, const Ghos
     const long nb = pblocks(d);
     const int gb = (G.ng + 255) / 256;
     if (mode == 0) {
-        hipLaunchKernelGGL((k_apply<T, 0>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
+        hipLaunchKernelGGL((k_apply<T, 0>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial, (T *)nullptr);
         if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 0>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
     } else if (mode == 1) {
-        hipLaunchKernelGGL((k_apply<T, 1>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
+        hipLaunchKernelGGL((k_apply<T, 1>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial, (T *)nullptr);
         if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 1>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
     } else {
-        hipLaunchKernelGGL((k_apply<T, 2>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial);
+        hipLaunchKernelGGL((k_apply<T, 2>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial, (T *)nullptr);
         if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 2>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
+    }
+}
+
+void launch_apply_f32in(hipStream_t st, const Dims &d, const SysCoef<double> &S, const GhostRows<double> &G,
+                        const uint8_t *flag, const float *x32, double *x, double *y, int mode, const double *w,
+                        double *partial) {
+    const long nb = pblocks(d);
+    const int gb = (G.ng + 255) / 256;
+    if (mode == 1) {
+        hipLaunchKernelGGL((k_apply<double, 1, float>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x32, y, w, partial, x);
+        if (gb)
+            hipLaunchKernelGGL((k_apply_ghost<double, 1>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
+    } else {
+        hipLaunchKernelGGL((k_apply<double, 2, float>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x32, y, w, partial, x);
+        if (gb)
+            hipLaunchKernelGGL((k_apply_ghost<double, 2>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
     }
 }
 
@@ -467,6 +515,57 @@ void launch_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const Gho
     if (gb) hipLaunchKernelGGL((k_jacobi_ghost<T>), dim3(gb), dim3(256), 0, st, G, S.is_p, r, z);
 }
 
+// BiCGSTAB direction update in FP64 (SJ 0: p = r + beta (p - omega v); SJ 1:
+// s = r - alpha v, into p) that also writes the FP32 right-hand side of the
+// multigrid preconditioner (b32 = (float) p) and, FIRST, the first smoother
+// colour on it (x32). Same arithmetic as k_bicg_p / k_bicg_s + k_d2f_first.
+template <int SJ, bool FIRST>
+__global__ void __launch_bounds__(256) k_bicg_d2f(Dims d, SysCoef<float> S, const uint8_t *flag, const KScal *ks,
+                                                  const double *r, const double *v, double *p, float *b32,
+                                                  float *x32) {
+    const int i0 = 2 * (blockIdx.x * 64 + threadIdx.x), j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i0 >= d.nx || j >= d.ny) return;
+    const long q0 = cidx(d, i0, j, k);
+    const double2 rv = *reinterpret_cast<const double2 *>(r + q0);
+    const double2 vv = *reinterpret_cast<const double2 *>(v + q0);
+    const unsigned fv = FIRST ? *reinterpret_cast<const unsigned short *>(flag + q0) : 0u;
+    double n0, n1;
+    if (SJ == 0) {
+        const double2 pv = *reinterpret_cast<const double2 *>(p + q0);
+        const double beta = ks->beta, omega = ks->omega;
+        n0 = rv.x + beta * (pv.x - omega * vv.x);
+        n1 = rv.y + beta * (pv.y - omega * vv.y);
+    } else {
+        const double alpha = ks->alpha;
+        n0 = rv.x - alpha * vv.x;
+        n1 = rv.y - alpha * vv.y;
+    }
+    const float b0 = (float)n0, b1 = (float)n1;
+    if (i0 + 1 < d.nx) {
+        *reinterpret_cast<double2 *>(p + q0) = make_double2(n0, n1);
+        *reinterpret_cast<float2 *>(b32 + q0) = make_float2(b0, b1);
+        if (FIRST)
+            *reinterpret_cast<float2 *>(x32 + q0) = make_float2(first_colour(S, fv & 0xFFu, i0, j, k, q0, b0),
+                                                                first_colour(S, fv >> 8, i0 + 1, j, k, q0 + 1, b1));
+    } else {
+        p[q0] = n0;
+        b32[q0] = b0;
+        if (FIRST) x32[q0] = first_colour(S, fv & 0xFFu, i0, j, k, q0, b0);
+    }
+}
+
+void launch_bicg_d2f(hipStream_t st, const Dims &d, const SysCoef<float> *S, const uint8_t *flag, const KScal *ks,
+                     int which, const double *r, const double *v, double *p, float *b32, float *x32) {
+    const SysCoef<float> S0 = S ? *S : SysCoef<float>{};
+    if (which == 0) {
+        if (x32) hipLaunchKernelGGL((k_bicg_d2f<0, true>), jgrid(d), dim3(64, 4), 0, st, d, S0, flag, ks, r, v, p, b32, x32);
+        else hipLaunchKernelGGL((k_bicg_d2f<0, false>), jgrid(d), dim3(64, 4), 0, st, d, S0, flag, ks, r, v, p, b32, x32);
+    } else {
+        if (x32) hipLaunchKernelGGL((k_bicg_d2f<1, true>), jgrid(d), dim3(64, 4), 0, st, d, S0, flag, ks, r, v, p, b32, x32);
+        else hipLaunchKernelGGL((k_bicg_d2f<1, false>), jgrid(d), dim3(64, 4), 0, st, d, S0, flag, ks, r, v, p, b32, x32);
+    }
+}
+
 template <typename T>
 void launch_bicg_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G,
                         const uint8_t *flag, const KScal *ks, int which, const T *r, const T *v, T *p, T *ph) {
@@ -563,40 +662,58 @@ bool launch_mg_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, cons
 // bc = (sum over fluid children of r * V_child) / V_coarse, r = b - A x
 // (pinned row: b - x). Equal to the residual kernel followed by the
 // restriction kernel, without storing r.
-template <typename T>
+template <typename T, bool FIRST>
 __global__ void __launch_bounds__(256) k_resid_restrict(Dims df, SysCoef<T> S, const uint8_t *flag_f, const T *b,
                                                         const T *x, Dims dc, MGTransfer tr, const double *dxf,
                                                         const double *dyf, const double *dzf, const double *dxc,
-                                                        const double *dyc, const double *dzc, T *bc) {
+                                                        const double *dyc, const double *dzc, T *bc,
+                                                        SysCoef<T> Sc, const uint8_t *flag_c, T *xc) {
     const int I = blockIdx.x * 64 + threadIdx.x, J = blockIdx.y * 4 + threadIdx.y, K = blockIdx.z;
     if (I >= dc.nx || J >= dc.ny) return;
+    // one or two children per axis; a single child is evaluated twice with
+    // the second copy weighted 0, and non-fluid children are weighted 0, so
+    // the loads of all eight children issue together (these levels are
+    // latency bound)
+    const int i0 = tr.si[I], ni = tr.si[I + 1] - i0, j0 = tr.sj[J], nj = tr.sj[J + 1] - j0;
+    const int k0 = tr.sk[K], nk = tr.sk[K + 1] - k0;
+    const int ii[2] = {i0, i0 + ni - 1}, jj[2] = {j0, j0 + nj - 1}, kk[2] = {k0, k0 + nk - 1};
+    const double vx[2] = {C1(dxf, ii[0]), ni > 1 ? C1(dxf, ii[1]) : 0.0};
+    const double vy[2] = {C1(dyf, jj[0]), nj > 1 ? C1(dyf, jj[1]) : 0.0};
+    const double vz[2] = {C1(dzf, kk[0]), nk > 1 ? C1(dzf, kk[1]) : 0.0};
+    T rv[8];
+    uint8_t fv[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int i = ii[c & 1], j = jj[(c >> 1) & 1], k = kk[c >> 2];
+        const long q = cidx(df, i, j, k);
+        fv[c] = flag_f[q];
+        const T bq = b[q], xq = x[q];
+        T diag, off;
+        stencil(df, S, flag_f, i, j, k, q, x, diag, off);
+        rv[c] = q == S.pin ? bq - xq : bq - (diag * xq - off);
+    }
     double acc = 0;
-    for (int k = tr.sk[K]; k < tr.sk[K + 1]; k++)
-        for (int j = tr.sj[J]; j < tr.sj[J + 1]; j++)
-            for (int i = tr.si[I]; i < tr.si[I + 1]; i++) {
-                const long q = cidx(df, i, j, k);
-                if (flag_f[q] != F_FLUID) continue;
-                T rv;
-                if (q == S.pin) {
-                    rv = b[q] - x[q];
-                } else {
-                    T diag, off;
-                    stencil(df, S, flag_f, i, j, k, q, x, diag, off);
-                    rv = b[q] - (diag * x[q] - off);
-                }
-                acc += (double)rv * C1(dxf, i) * C1(dyf, j) * C1(dzf, k);
-            }
-    bc[cidx(dc, I, J, K)] = (T)(acc / (C1(dxc, I) * C1(dyc, J) * C1(dzc, K)));
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+        if (fv[c] == F_FLUID) acc += (double)rv[c] * vx[c & 1] * vy[(c >> 1) & 1] * vz[c >> 2];
+    const long qc = cidx(dc, I, J, K);
+    const T bv = (T)(acc / (C1(dxc, I) * C1(dyc, J) * C1(dzc, K)));
+    bc[qc] = bv;
+    if (FIRST) xc[qc] = first_colour(Sc, flag_c[qc], I, J, K, qc, bv);
 }
 
 template <typename T>
 void launch_resid_restrict(hipStream_t st, const Dims &df, const SysCoef<T> &S, const uint8_t *flag_f, const T *b,
                            const T *x, const Dims &dc, const MGTransfer &tr, const double *dxyz_f,
-                           const double *dxyz_c, T *bc) {
+                           const double *dxyz_c, T *bc, const SysCoef<T> *Sc, const uint8_t *flag_c, T *xc) {
     const double *dxf = dxyz_f, *dyf = dxf + df.nx + 4, *dzf = dyf + df.ny + 4;
     const double *dxc = dxyz_c, *dyc = dxc + dc.nx + 4, *dzc = dyc + dc.ny + 4;
-    hipLaunchKernelGGL((k_resid_restrict<T>), igrid(dc), dim3(64, 4), 0, st, df, S, flag_f, b, x, dc, tr, dxf, dyf,
-                       dzf, dxc, dyc, dzc, bc);
+    if (xc)
+        hipLaunchKernelGGL((k_resid_restrict<T, true>), igrid(dc), dim3(64, 4), 0, st, df, S, flag_f, b, x, dc, tr, dxf,
+                           dyf, dzf, dxc, dyc, dzc, bc, *Sc, flag_c, xc);
+    else
+        hipLaunchKernelGGL((k_resid_restrict<T, false>), igrid(dc), dim3(64, 4), 0, st, df, S, flag_f, b, x, dc, tr,
+                           dxf, dyf, dzf, dxc, dyc, dzc, bc, S, flag_c, xc);
 }
 
 // planes per workgroup: enough workgroups to fill the chip
@@ -1042,6 +1159,26 @@ __global__ void __launch_bounds__(256) k_d2f(Dims d, const double *a, float *b) 
     const long q = cidx(d, i, j, k);
     b[q] = (float)a[q];
 }
+// b = (float) a, and x = the first smoother colour from a zero start; two
+// cells of a row per lane
+__global__ void __launch_bounds__(256) k_d2f_first(Dims d, SysCoef<float> S, const uint8_t *flag, const double *a,
+                                                   float *b, float *x) {
+    const int i0 = 2 * (blockIdx.x * 64 + threadIdx.x), j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i0 >= d.nx || j >= d.ny) return;
+    const long q0 = cidx(d, i0, j, k);
+    const double2 av = *reinterpret_cast<const double2 *>(a + q0);
+    const unsigned fv = *reinterpret_cast<const unsigned short *>(flag + q0);
+    const float b0 = (float)av.x, b1 = (float)av.y;
+    const float x0 = first_colour(S, fv & 0xFFu, i0, j, k, q0, b0);
+    if (i0 + 1 < d.nx) {
+        const float x1 = first_colour(S, fv >> 8, i0 + 1, j, k, q0 + 1, b1);
+        *reinterpret_cast<float2 *>(b + q0) = make_float2(b0, b1);
+        *reinterpret_cast<float2 *>(x + q0) = make_float2(x0, x1);
+    } else {
+        b[q0] = b0;
+        x[q0] = x0;
+    }
+}
 __global__ void __launch_bounds__(256) k_add_f2d(Dims d, const float *e, double *x) {
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
     if (i >= d.nx || j >= d.ny) return;
@@ -1070,6 +1207,10 @@ __global__ void __launch_bounds__(256) k_copy(Dims d, const T *a, T *b) {
 void launch_d2f(hipStream_t st, const Dims &d, const double *a, float *b) {
     hipLaunchKernelGGL(k_d2f, igrid(d), dim3(64, 4), 0, st, d, a, b);
 }
+void launch_d2f_first(hipStream_t st, const Dims &d, const SysCoef<float> &S, const uint8_t *flag, const double *a,
+                      float *b, float *x) {
+    hipLaunchKernelGGL(k_d2f_first, jgrid(d), dim3(64, 4), 0, st, d, S, flag, a, b, x);
+}
 void launch_add_f2d(hipStream_t st, const Dims &d, const float *e, double *x) {
     hipLaunchKernelGGL(k_add_f2d, igrid(d), dim3(64, 4), 0, st, d, e, x);
 }
@@ -1089,68 +1230,109 @@ void launch_copy(hipStream_t st, const Dims &d, const T *a, T *b) {
 // multigrid transfers
 // ---------------------------------------------------------------------------
 // coarse rhs = volume-weighted sum of the fluid children's residuals / coarse volume
-template <typename T, bool FLAGS>
+constexpr int kRestZ = 2;
+template <typename T, bool FLAGS, bool FIRST>
 __global__ void __launch_bounds__(256) k_restrict(Dims df, Dims dc, MGTransfer tr, const uint8_t *flag_f,
                                                   const T *rf, const double *dxf, const double *dyf,
                                                   const double *dzf, const double *dxc, const double *dyc,
-                                                  const double *dzc, T *bc) {
-    const int I = blockIdx.x * 64 + threadIdx.x, J = blockIdx.y * 4 + threadIdx.y, K = blockIdx.z;
+                                                  const double *dzc, T *bc, SysCoef<T> Sc, const uint8_t *flag_c,
+                                                  T *xc) {
+    // kRestZ coarse planes per lane (shared x / y child maps, all loads
+    // issued together)
+    const int I = blockIdx.x * 64 + threadIdx.x, J = blockIdx.y * 4 + threadIdx.y;
     if (I >= dc.nx || J >= dc.ny) return;
+    const int K0 = blockIdx.z * kRestZ;
     // one or two children per axis: a single child is read twice with the
     // second copy weighted 0, so all eight loads issue without branches
     const int i0 = tr.si[I], ni = tr.si[I + 1] - i0, j0 = tr.sj[J], nj = tr.sj[J + 1] - j0;
-    const int k0 = tr.sk[K], nk = tr.sk[K + 1] - k0;
-    const int ii[2] = {i0, i0 + ni - 1}, jj[2] = {j0, j0 + nj - 1}, kk[2] = {k0, k0 + nk - 1};
+    const int ii[2] = {i0, i0 + ni - 1}, jj[2] = {j0, j0 + nj - 1};
     const double vx[2] = {C1(dxf, ii[0]), ni > 1 ? C1(dxf, ii[1]) : 0.0};
     const double vy[2] = {C1(dyf, jj[0]), nj > 1 ? C1(dyf, jj[1]) : 0.0};
-    const double vz[2] = {C1(dzf, kk[0]), nk > 1 ? C1(dzf, kk[1]) : 0.0};
-    T rv[8];
-    uint8_t fv[8];
+    T rv[kRestZ][8];
+    uint8_t fv[kRestZ][8];
+    double vz[kRestZ][2];
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const long q = cidx(df, ii[c & 1], jj[(c >> 1) & 1], kk[c >> 2]);
-        rv[c] = rf[q];
-        fv[c] = FLAGS ? flag_f[q] : (uint8_t)F_FLUID;  // !FLAGS: rf is 0 at non-fluid cells
+    for (int z = 0; z < kRestZ; z++) {
+        const int K = min(K0 + z, dc.nz - 1);  // past the last plane: a repeat, not written
+        const int k0 = tr.sk[K], nk = tr.sk[K + 1] - k0;
+        const int kk[2] = {k0, k0 + nk - 1};
+        vz[z][0] = C1(dzf, kk[0]);
+        vz[z][1] = nk > 1 ? C1(dzf, kk[1]) : 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const long q = cidx(df, ii[c & 1], jj[(c >> 1) & 1], kk[c >> 2]);
+            rv[z][c] = rf[q];
+            fv[z][c] = FLAGS ? flag_f[q] : (uint8_t)F_FLUID;  // !FLAGS: rf is 0 at non-fluid cells
+        }
     }
-    double acc = 0;
 #pragma unroll
-    for (int c = 0; c < 8; c++)
-        if (fv[c] == F_FLUID) acc += (double)rv[c] * vx[c & 1] * vy[(c >> 1) & 1] * vz[c >> 2];
-    bc[cidx(dc, I, J, K)] = (T)(acc / (C1(dxc, I) * C1(dyc, J) * C1(dzc, K)));
+    for (int z = 0; z < kRestZ; z++) {
+        const int K = K0 + z;
+        if (K >= dc.nz) break;
+        double acc = 0;
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            if (fv[z][c] == F_FLUID) acc += (double)rv[z][c] * vx[c & 1] * vy[(c >> 1) & 1] * vz[z][c >> 2];
+        const long qc = cidx(dc, I, J, K);
+        const T bv = (T)(acc / (C1(dxc, I) * C1(dyc, J) * C1(dzc, K)));
+        bc[qc] = bv;
+        if (FIRST) xc[qc] = first_colour(Sc, flag_c[qc], I, J, K, qc, bv);
+    }
 }
 
 // x_fine += trilinear interpolation of the coarse correction (fluid cells)
-template <typename T>
+// Each lane corrects KZ planes of one (i, j) column: the index and weight
+// loads are shared and all planes' loads issue together (on the large
+// levels, one plane per lane left the dependent index -> coarse-value loads
+// latency bound; small levels keep one plane per lane for parallelism).
+constexpr long kProlManyCells = 1L << 20;
+template <typename T, int KZ>
 __global__ void __launch_bounds__(256) k_prolong(Dims df, Dims dc, MGTransfer tr, const uint8_t *flag_f,
                                                  const uint8_t *flag_c, const T *ec, T *xf) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
     if (i >= df.nx || j >= df.ny) return;
-    const long q = cidx(df, i, j, k);
-    if (flag_f[q] != F_FLUID) return;
-    const int I = tr.ci[i], J = tr.cj[j], K = tr.ck[k];
-    const int Is[2] = {I, tr.pi[i]}, Js[2] = {J, tr.pj[j]}, Ks[2] = {K, tr.pk[k]};
-    const double wi[2] = {tr.wi[i], 1 - tr.wi[i]}, wj[2] = {tr.wj[j], 1 - tr.wj[j]}, wk[2] = {tr.wk[k], 1 - tr.wk[k]};
-    // the far neighbours are interior coarse cells (the parent itself where
-    // the weight is 0 or at non-periodic ends), so all eight loads issue
-    // without branches
-    // coarse flags are read only where the stencil touches an inactive cell
-    const bool chk = tr.pm[q] != 0;
-    T ev[8];
-    uint8_t fv[8];
+    const int k0 = blockIdx.z * KZ;
+    const int I = tr.ci[i], J = tr.cj[j];
+    const int Is[2] = {I, tr.pi[i]}, Js[2] = {J, tr.pj[j]};
+    const double wi[2] = {tr.wi[i], 1 - tr.wi[i]}, wj[2] = {tr.wj[j], 1 - tr.wj[j]};
+    bool fluid[KZ];
+    T xv[KZ], ev[KZ][8];
+    uint8_t fv[KZ][8];
+    double wk[KZ][2];
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const long qc = cidx(dc, Is[c & 1], Js[(c >> 1) & 1], Ks[c >> 2]);
-        ev[c] = ec[qc];
-        fv[c] = chk ? flag_c[qc] : (uint8_t)F_FLUID;
-    }
-    const double e0 = (double)ev[0];
-    double e = 0;
+    for (int z = 0; z < KZ; z++) {
+        const int k = min(k0 + z, df.nz - 1);  // past the last plane: a repeat, not written
+        const long q = cidx(df, i, j, k);
+        // the loads of every cell issue before the fluid test (non-fluid
+        // cells are not written); the far neighbours are interior coarse
+        // cells (the parent itself where the weight is 0 or at
+        // non-periodic ends), so all eight loads issue without branches;
+        // coarse flags are read only where the stencil touches an inactive
+        // cell
+        fluid[z] = flag_f[q] == F_FLUID && k0 + z < df.nz;
+        xv[z] = xf[q];
+        const int Ks[2] = {tr.ck[k], tr.pk[k]};
+        wk[z][0] = tr.wk[k];
+        wk[z][1] = 1 - wk[z][0];
+        const bool chk = tr.pm[q] != 0;
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const double w = wi[c & 1] * wj[(c >> 1) & 1] * wk[c >> 2];
-        if (w != 0) e += w * (fv[c] == F_SOLID ? e0 : (double)ev[c]);
+        for (int c = 0; c < 8; c++) {
+            const long qc = cidx(dc, Is[c & 1], Js[(c >> 1) & 1], Ks[c >> 2]);
+            ev[z][c] = ec[qc];
+            fv[z][c] = chk ? flag_c[qc] : (uint8_t)F_FLUID;
+        }
     }
-    xf[q] += (T)e;
+#pragma unroll
+    for (int z = 0; z < KZ; z++) {
+        const double e0 = (double)ev[z][0];
+        double e = 0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const double w = wi[c & 1] * wj[(c >> 1) & 1] * wk[z][c >> 2];
+            if (w != 0) e += w * (fv[z][c] == F_SOLID ? e0 : (double)ev[z][c]);
+        }
+        if (fluid[z]) xf[cidx(df, i, j, k0 + z)] = xv[z] + (T)e;
+    }
 }
 
 // single-workgroup red-black Gauss-Seidel on the coarsest grid; for a
@@ -1229,18 +1411,31 @@ __global__ void __launch_bounds__(1024) k_coarse(Dims d, SysCoef<T> S, const uin
 template <typename T>
 void launch_restrict(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
                      const T *rf, const double *dxf, const double *dyf, const double *dzf, const double *dxc,
-                     const double *dyc, const double *dzc, T *bc, bool rf_zero_off_fluid) {
-    if (rf_zero_off_fluid)
-        hipLaunchKernelGGL((k_restrict<T, false>), igrid(dc), dim3(64, 4), 0, st, df, dc, tr, flag_f, rf, dxf, dyf, dzf,
-                           dxc, dyc, dzc, bc);
-    else
-        hipLaunchKernelGGL((k_restrict<T, true>), igrid(dc), dim3(64, 4), 0, st, df, dc, tr, flag_f, rf, dxf, dyf, dzf,
-                           dxc, dyc, dzc, bc);
+                     const double *dyc, const double *dzc, T *bc, bool rf_zero_off_fluid, const SysCoef<T> *Sc,
+                     const uint8_t *flag_c, T *xc) {
+    const dim3 grid((dc.nx + 63) / 64, (dc.ny + 3) / 4, (dc.nz + kRestZ - 1) / kRestZ);
+    const SysCoef<T> S0 = Sc ? *Sc : SysCoef<T>{};
+#define RESTRICT_LAUNCH(FL, FI)                                                                                 \
+    hipLaunchKernelGGL((k_restrict<T, FL, FI>), grid, dim3(64, 4), 0, st, df, dc, tr, flag_f, rf, dxf, dyf, dzf, \
+                       dxc, dyc, dzc, bc, S0, flag_c, xc)
+    if (rf_zero_off_fluid) {
+        if (xc) RESTRICT_LAUNCH(false, true);
+        else RESTRICT_LAUNCH(false, false);
+    } else {
+        if (xc) RESTRICT_LAUNCH(true, true);
+        else RESTRICT_LAUNCH(true, false);
+    }
+#undef RESTRICT_LAUNCH
 }
 template <typename T>
 void launch_prolong(hipStream_t st, const Dims &df, const Dims &dc, const MGTransfer &tr, const uint8_t *flag_f,
                     const uint8_t *flag_c, const T *ec, T *xf) {
-    hipLaunchKernelGGL((k_prolong<T>), igrid(df), dim3(64, 4), 0, st, df, dc, tr, flag_f, flag_c, ec, xf);
+    if ((long)df.nx * df.ny * df.nz >= kProlManyCells) {
+        const dim3 grid((df.nx + 63) / 64, (df.ny + 3) / 4, (df.nz + 3) / 4);
+        hipLaunchKernelGGL((k_prolong<T, 4>), grid, dim3(64, 4), 0, st, df, dc, tr, flag_f, flag_c, ec, xf);
+    } else {
+        hipLaunchKernelGGL((k_prolong<T, 1>), igrid(df), dim3(64, 4), 0, st, df, dc, tr, flag_f, flag_c, ec, xf);
+    }
 }
 template <typename T>
 void launch_coarse_solve(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b, T *x,
@@ -1360,7 +1555,7 @@ void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, 
                                         const T *, T *);                                                          \
     template void launch_resid_restrict<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *,         \
                                            const T *, const T *, const Dims &, const MGTransfer &, const double *, \
-                                           const double *, T *);                                                  \
+                                           const double *, T *, const SysCoef<T> *, const uint8_t *, T *);        \
     template int launch_ghost_relax<T>(hipStream_t, const GhostRows<T> &, int, const T *, T *, int);               \
     template void launch_apply<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,             \
                                   const uint8_t *, const T *, T *, int, const T *, double *);                      \
@@ -1380,7 +1575,8 @@ void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, 
     template void launch_copy<T>(hipStream_t, const Dims &, const T *, T *);                                       \
     template void launch_restrict<T>(hipStream_t, const Dims &, const Dims &, const MGTransfer &, const uint8_t *, \
                                      const T *, const double *, const double *, const double *, const double *,    \
-                                     const double *, const double *, T *, bool);                                   \
+                                     const double *, const double *, T *, bool, const SysCoef<T> *,                \
+                                     const uint8_t *, T *);                                                        \
     template void launch_prolong<T>(hipStream_t, const Dims &, const Dims &, const MGTransfer &, const uint8_t *,  \
                                     const uint8_t *, const T *, T *);                                              \
     template void launch_coarse_solve<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *,           \

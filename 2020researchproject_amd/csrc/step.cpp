@@ -550,7 +550,7 @@ static const int kCoarse = 40;
 // written on half of the cells), ghost rows relaxed after each colour.
 template <typename T>
 static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
-                   const T *b, T *x, T *xalt, int sweeps, bool zero_init, int level) {
+                   const T *b, T *x, T *xalt, int sweeps, bool zero_init, int level, bool first_done = false) {
     const double n = (double)d.nx * d.ny * d.nz;
     char nm[32];
     // a zero start also zeroes the ghost values (s->ghost_prev = -1)
@@ -564,11 +564,13 @@ static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const Ghost
     if (s->mg_smoother == 1) {
         for (int sw = 0; sw < sweeps; sw++)
             for (int c = 0; c < 2; c++) {
-                // the first colour from a zero start writes every cell
+                // the first colour from a zero start writes every cell;
+                // first_done: the kernel that wrote b already wrote it
                 const bool z = zero_init && sw == 0 && c == 0;
                 snprintf(nm, sizeof nm, "rbgs_L%d", level);
-                TIMED(nm, z ? n * (1.5 * sizeof(T) + 1) : n * (2 * sizeof(T) + 1),
-                      launch_rbgs_color<T>(s->stream, d, S, flag, b, x, c, z ? 1 : 0));
+                if (!(z && first_done))
+                    TIMED(nm, z ? n * (1.5 * sizeof(T) + 1) : n * (2 * sizeof(T) + 1),
+                          launch_rbgs_color<T>(s->stream, d, S, flag, b, x, c, z ? 1 : 0));
                 if (!G.ng) continue;
                 if (z) s->ghost_prev = -1;
                 TIMED("ghost_relax", 0, s->ghost_prev = launch_ghost_relax<T>(s->stream, G, S.is_p, b, x, s->ghost_prev));
@@ -590,8 +592,19 @@ static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const Ghost
     if (cur != x) TIMED("mg_copy", 2 * n * sizeof(T), launch_copy<T>(s->stream, d, cur, x));
 }
 
+// Whether the kernel that produces level m's right-hand side also writes the
+// first colour of its pre-smoothing (one-launch-per-colour smoother, a
+// smoothed level, at least one pre-sweep).
+static bool fuse_first_colour(const IBMSolver *s, int m) {
+    if (s->mg_smoother != 1 || s->mg_pre < 1) return false;
+    if (m >= 1 && s->mg[m].dn > 0) return false;       // dense coarsest solve
+    if (m + 1 == (int)s->mg.size()) return false;      // iterative coarsest solve
+    return true;
+}
+
+// first_done: x already holds the first colour of the pre-smoothing
 template <typename T>
-static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
+static void vcycle(IBMSolver *s, int l, const T *b, T *x, bool first_done = false) {
     MGLevel &L = s->mg[l];
     const bool fine = l == 0;
     const SysCoef<T> &S = fine ? sysc<T>(s->sys[3]) : sysc<T>(L.sys);
@@ -610,8 +623,12 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
         return;
     }
     const bool light = s->mg_light_from > 0 && l >= s->mg_light_from;
-    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, light ? 1 : s->mg_pre, true, l);
+    smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, light ? 1 : s->mg_pre, true, l, first_done);
     MGLevel &Cl = s->mg[l + 1];
+    const bool fuse = fuse_first_colour(s, l + 1);
+    const SysCoef<T> *Sc = fuse ? &sysc<T>(Cl.sys) : nullptr;
+    const uint8_t *flag_c = fuse ? Cl.flag.p : nullptr;
+    T *xc = fuse ? lev_x<T>(Cl).p : nullptr;
     if ((long)L.d.nx * L.d.ny * L.d.nz >= kSplitRestrictCells) {
         // large levels: residual, then restriction (measured faster than the
         // one-pass kernel, whose per-coarse-cell child loops run short here)
@@ -624,16 +641,16 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x) {
         TIMED(tn, cell_b + cell_b / sizeof(T),
               launch_restrict<T>(s->stream, L.d, Cl.d, Cl.tr, flag, lev_r<T>(L).p, L.dxyz.p, L.dxyz.p + (L.d.nx + 4),
                                  L.dxyz.p + (L.d.nx + 4) + (L.d.ny + 4), Cl.dxyz.p, Cl.dxyz.p + (Cl.d.nx + 4),
-                                 Cl.dxyz.p + (Cl.d.nx + 4) + (Cl.d.ny + 4), lev_b<T>(Cl).p, rz));
+                                 Cl.dxyz.p + (Cl.d.nx + 4) + (Cl.d.ny + 4), lev_b<T>(Cl).p, rz, Sc, flag_c, xc));
     } else {
         // residual restricted in one pass: x and b read, one flag byte per cell
         char nm[32];
         snprintf(nm, sizeof nm, "mg_resid_restrict_L%d", l);
         TIMED(nm, 2 * cell_b + cell_b / sizeof(T),
               launch_resid_restrict<T>(s->stream, L.d, S, flag, b, x, Cl.d, Cl.tr, L.dxyz.p, Cl.dxyz.p,
-                                       lev_b<T>(Cl).p));
+                                       lev_b<T>(Cl).p, Sc, flag_c, xc));
     }
-    vcycle<T>(s, l + 1, lev_b<T>(Cl).p, lev_x<T>(Cl).p);
+    vcycle<T>(s, l + 1, lev_b<T>(Cl).p, lev_x<T>(Cl).p, fuse);
     char pnm[32];
     snprintf(pnm, sizeof pnm, "mg_prolong_L%d", l);
     // fine x read and written, one flag byte per fine cell (coarse reads cached)
@@ -684,10 +701,23 @@ static void mg_precond(IBMSolver *s, const T *in, T *out) {
 static void mg_precond_f32(IBMSolver *s, const double *in, double *out) {
     const double cb = ncells(s);
     replay_or_capture(s, in, out, [&]() {
-        TIMED("d2f", cb * 12, launch_d2f(s->stream, s->d, in, s->ref_rhs32.p));
-        vcycle<float>(s, 0, s->ref_rhs32.p, s->ref_e32.p);
+        // the conversion also writes the first smoother colour of the fine level
+        const bool fuse = fuse_first_colour(s, 0);
+        if (fuse)
+            TIMED("d2f", cb * 17, launch_d2f_first(s->stream, s->d, s->sys[3].S32, s->flag.p, in, s->ref_rhs32.p,
+                                                   s->ref_e32.p));
+        else
+            TIMED("d2f", cb * 12, launch_d2f(s->stream, s->d, in, s->ref_rhs32.p));
+        vcycle<float>(s, 0, s->ref_rhs32.p, s->ref_e32.p, fuse);
         TIMED("f2d", cb * 12, launch_f2d(s->stream, s->d, s->ref_e32.p, out));
     });
+}
+
+// The FP32 V-cycle on ref_rhs32 into ref_e32 alone (its conversions fused
+// into the neighbouring Krylov kernels); first_done as for vcycle.
+static void mg_vcycle_f32(IBMSolver *s, bool first_done) {
+    replay_or_capture(s, s->ref_rhs32.p, s->ref_e32.p,
+                      [&]() { vcycle<float>(s, 0, s->ref_rhs32.p, s->ref_e32.p, first_done); });
 }
 
 // ---------------------------------------------------------------------------
@@ -727,27 +757,62 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
             launch_copy<T>(s->stream, d, in, out);
         }
     };
+    // FP32 multigrid preconditioner of the FP64 iteration: conversions fused
+    const bool f32mg = prec == 3 && std::is_same<T, double>::value;
+    const bool fuse1 = f32mg && fuse_first_colour(s, 0);
+    const SysCoef<float> *S32f = fuse1 ? &s->sys[3].S32 : nullptr;
+    float *x32f = fuse1 ? s->ref_e32.p : nullptr;
     int it = 0;
     for (;;) {
     while (it < maxit && rn > tol * bn) {
         it++;
-        snprintf(nm, sizeof nm, "bicg_vec");
+        snprintf(nm, sizeof nm, "spmv_%s", tag);
+        // direction p and ph = M^-1 p, then v = A ph. FP32 multigrid: the
+        // update writes the FP32 right-hand side (and the cycle's first
+        // colour), the operator reads the FP32 result and writes ph.
         if (prec == 1) {
             TIMED("bicg_jacobi", 5 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 0, r, v, p, ph));
+        } else if (f32mg) {
+            if constexpr (std::is_same<T, double>::value) {
+                TIMED("bicg_d2f", (fuse1 ? 5.5 : 5) * cb, launch_bicg_d2f(s->stream, d, S32f, flag, s->ks.p, 0, r, v, p,
+                                                                     s->ref_rhs32.p, x32f));
+                mg_vcycle_f32(s, fuse1);
+            }
         } else {
             TIMED("bicg_vec", 4 * cb, launch_bicg_p<T>(s->stream, d, s->ks.p, r, v, p));
             precond(p, ph);
         }
-        snprintf(nm, sizeof nm, "spmv_%s", tag);
-        TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, ph, v, 1, rh, s->partial.p));
+        if constexpr (std::is_same<T, double>::value) {
+            if (f32mg)
+                TIMED(nm, 3.5 * cb, launch_apply_f32in(s->stream, d, S, G, flag, s->ref_e32.p, ph, v, 1, rh,
+                                                       s->partial.p));
+            else
+                TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, ph, v, 1, rh, s->partial.p));
+        } else {
+            TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, ph, v, 1, rh, s->partial.p));
+        }
         launch_finalize(s->stream, 0, s->partial.p, na, 1, s->ks.p);
         if (prec == 1) {
             TIMED("bicg_jacobi", 4 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 1, r, v, sv, sh));
+        } else if (f32mg) {
+            if constexpr (std::is_same<T, double>::value) {
+                TIMED("bicg_d2f", (fuse1 ? 4.5 : 4) * cb, launch_bicg_d2f(s->stream, d, S32f, flag, s->ks.p, 1, r, v, sv,
+                                                                     s->ref_rhs32.p, x32f));
+                mg_vcycle_f32(s, fuse1);
+            }
         } else {
             TIMED("bicg_vec", 3 * cb, launch_bicg_s<T>(s->stream, d, s->ks.p, r, v, sv));
             precond(sv, sh);
         }
-        TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, sh, t, 2, sv, s->partial.p));
+        if constexpr (std::is_same<T, double>::value) {
+            if (f32mg)
+                TIMED(nm, 3.5 * cb, launch_apply_f32in(s->stream, d, S, G, flag, s->ref_e32.p, sh, t, 2, sv,
+                                                       s->partial.p));
+            else
+                TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, sh, t, 2, sv, s->partial.p));
+        } else {
+            TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, sh, t, 2, sv, s->partial.p));
+        }
         launch_finalize(s->stream, 1, s->partial.p, na, 2, s->ks.p);
         TIMED("bicg_xr", 8 * cb, launch_bicg_xr<T>(s->stream, d, s->ks.p, ph, sh, sv, t, rh, x, r, ns, s->partial.p));
         finalize_read(s, 2, nv, 3);
