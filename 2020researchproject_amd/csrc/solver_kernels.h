@@ -81,9 +81,18 @@ void launch_rbgs_fused(hipStream_t st, const Dims &d, const SysCoef<T> &S, const
 // Jacobi relaxation of the ghost rows in one launch. `prev` names where the
 // current ghost values are: -1 all zero, 0 or 1 G.vals[prev] (they equal x
 // at the ghost cells). The new values go to x and to the other buffer, whose
-// index is returned: the next relaxation's `prev`.
+// index is returned: the next relaxation's `prev`. only >= 0: only the rows
+// whose cell (i, j, k) has i + j + k % 2 == only are relaxed (the others
+// carry their values over); d is then the grid of the cells.
 template <typename T>
-int launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x, int prev);
+int launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x, int prev,
+                       Dims d = Dims{}, int only = -1);
+// One colour (not from a zero start) followed by the relaxation of that
+// colour's ghost rows (launch_ghost_relax with only = color), in one launch;
+// prev >= 0 as for launch_ghost_relax, whose return value it shares.
+template <typename T>
+int launch_rbgs_color_ghost(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
+                            T *x, int color, const GhostRows<T> &G, int prev);
 
 // BiCGSTAB pieces
 template <typename T>

@@ -582,9 +582,8 @@ void launch_bicg_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, cons
 // Thread t of a row handles x = 2t + parity so every lane owns one cell.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_rbgs(Dims d, SysCoef<T> S, const uint8_t *flag, const T *b, T *x,
-                                              int color, int zero_in, int xcd) {
-    const uint3 bk = xcd_block(xcd);
+__device__ inline void rbgs_cell(const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b, T *x,
+                                 int color, int zero_in, uint3 bk) {
     const int j = bk.y * 4 + threadIdx.y, k = bk.z;
     const int i = 2 * (bk.x * 64 + threadIdx.x) + ((j + k + color) & 1);
     if (i >= d.nx || j >= d.ny) return;
@@ -612,6 +611,103 @@ __global__ void __launch_bounds__(256) k_rbgs(Dims d, SysCoef<T> S, const uint8_
     x[q] = diag != T(0) ? (bv + off) / diag : T(0);
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k_rbgs(Dims d, SysCoef<T> S, const uint8_t *flag, const T *b, T *x,
+                                              int color, int zero_in, int xcd) {
+    rbgs_cell(d, S, flag, b, x, color, zero_in, xcd_block(xcd));
+}
+
+// (i, j, k) of a linear index of the padded layout (any cell, ghost layers
+// included; 32-bit arithmetic: padded grids stay below 2^31 cells)
+__device__ inline void cell_ijk(const Dims &d, long q, int &i, int &j, int &k) {
+    const unsigned r = (unsigned)(q - (d.off - 2 - 2 * (long)d.sy - 2 * d.sz));  // cell (-2, -2, -2) -> 0
+    const unsigned sy = (unsigned)d.sy, sz = (unsigned)d.sz;
+    const unsigned kz = r / sz, rem = r - kz * sz, jy = rem / sy;
+    k = (int)kz - 2;
+    j = (int)jy - 2;
+    i = (int)(rem - jy * sy) - 2;
+}
+
+// Ghost row s relaxed inside the smoother launch of colour `color` (not a
+// zero start). Rows whose cell has that colour take the Jacobi value of their
+// row from the current sources, the rest carry their value over (next =
+// prev). A fluid source of the launch's colour is updated by the same launch,
+// so its new value is evaluated here with the smoother's own arithmetic (its
+// neighbours are of the other colour, which the launch does not change);
+// fluid cells never read a ghost cell of the launch's colour. The result
+// equals a colour launch followed by a relaxation of that colour's ghost rows.
+// All eight sources' loads (their stencils included: a source that needs no
+// stencil evaluates the ghost cell's, whose neighbours are in range) issue
+// together.
+template <typename T>
+__device__ inline void ghost_row_in_colour(const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
+                                           T *x, const GhostRows<T> &G, int is_p, const T *prev, T *next,
+                                           int color, int s) {
+    const long c = G.cell[s];
+    int ic, jc, kc;
+    cell_ijk(d, c, ic, jc, kc);
+    if (((ic + jc + kc) & 1) != color) {
+        next[s] = prev[s];
+        return;
+    }
+    T vals[8];
+    bool used[8];
+    T wv[8];
+#pragma unroll
+    for (int l = 0; l < 8; l++) {
+        const long q0 = G.src[8 * (long)s + l];
+        const int g = G.srcg[8 * (long)s + l];
+        wv[l] = G.w[8 * (long)s + l];
+        used[l] = q0 >= 0;
+        const long q = q0 >= 0 ? q0 : c;
+        int i, j, k;
+        cell_ijk(d, q, i, j, k);
+        const bool same = g < 0 && i >= 0 && i < d.nx && j >= 0 && j < d.ny && k >= 0 && k < d.nz &&
+                          ((i + j + k) & 1) == color;
+        const long qs = same ? q : c;
+        const uint8_t f = flag[q];
+        const T bq = b[q], xq = x[q], pg = prev[g >= 0 ? g : 0];
+        T diag, off;
+        stencil(d, S, flag, same ? i : ic, same ? j : jc, same ? k : kc, qs, x, diag, off);
+        T v = xq;
+        if (g >= 0) v = pg;
+        else if (same && (q == S.pin || f == F_SOLID)) v = bq;
+        else if (same && f == F_FLUID) v = diag != T(0) ? (bq + off) / diag : T(0);
+        vals[l] = v;
+    }
+    T a = 0;
+#pragma unroll
+    for (int l = 0; l < 8; l++)
+        if (used[l]) a += wv[l] * vals[l];
+    const T ws = G.wself[s];
+    T v;
+    if (is_p) {
+        const T dg = T(1) - ws;
+        v = dg != T(0) ? (b[c] + a) / dg : x[c];
+    } else {
+        v = (b[c] - a) / (T(1) + ws);
+    }
+    x[c] = v;
+    next[s] = v;
+}
+
+// one colour (not a zero start) with that colour's ghost rows: the first gz
+// planes of blocks relax ghost rows (64 per block), the others run the
+// one-colour sweep
+template <typename T>
+__global__ void __launch_bounds__(256) k_rbgs_g(Dims d, SysCoef<T> S, const uint8_t *flag, const T *b, T *x,
+                                                int color, GhostRows<T> G, int is_p, const T *prev, T *next, int gz) {
+    uint3 bk = xcd_block(1);
+    if ((int)bk.z < gz) {
+        // one wave of rows per block: the rows' gathers spread over many CUs
+        const int s = (int)(((long)bk.z * gridDim.y + bk.y) * gridDim.x + bk.x) * 64 + threadIdx.x;
+        if (threadIdx.y == 0 && s < G.ng) ghost_row_in_colour(d, S, flag, b, x, G, is_p, prev, next, color, s);
+        return;
+    }
+    bk.z -= gz;
+    rbgs_cell(d, S, flag, b, x, color, 0, bk);
+}
+
 // four cells per lane (rbgs_quad.h) on levels with rows of at least this
 // many cells; shorter rows leave most lanes of a 64-lane row idle
 constexpr int kQuadMinRow = 128;
@@ -620,6 +716,21 @@ template <int BY>
 __global__ void __launch_bounds__(64 * BY) k_rbgs_quad(Dims d, SysCoef<float> S, const uint8_t *flag, const float *b,
                                                        float *x, int color, int zero_in) {
     rbgs_quad<BY>(d, S, flag, b, x, color, zero_in, xcd_block(1));
+}
+
+template <int BY>
+__global__ void __launch_bounds__(64 * BY) k_rbgs_quad_g(Dims d, SysCoef<float> S, const uint8_t *flag,
+                                                         const float *b, float *x, int color, GhostRows<float> G,
+                                                         int is_p, const float *prev, float *next, int gz) {
+    uint3 bk = xcd_block(1);
+    if ((int)bk.z < gz) {
+        // one wave of rows per block: the rows' gathers spread over many CUs
+        const int s = (int)(((long)bk.z * gridDim.y + bk.y) * gridDim.x + bk.x) * 64 + threadIdx.x;
+        if (threadIdx.y == 0 && s < G.ng) ghost_row_in_colour(d, S, flag, b, x, G, is_p, prev, next, color, s);
+        return;
+    }
+    bk.z -= gz;
+    rbgs_quad<BY>(d, S, flag, b, x, color, 0, bk);
 }
 
 // r = b - A x on the fluid rows (pinned row: b - x); no norms
@@ -738,9 +849,18 @@ void launch_rbgs_fused(hipStream_t st, const Dims &d, const SysCoef<T> &S, const
 // writes x at ghost cells in between), so the result does not depend on the
 // order in which waves run.
 template <typename T>
-__global__ void k_ghost_relax(GhostRows<T> G, int is_p, const T *b, T *x, const T *prev, T *next) {
+__global__ void k_ghost_relax(GhostRows<T> G, int is_p, const T *b, T *x, const T *prev, T *next, Dims d, int only) {
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= G.ng) return;
+    if (only >= 0) {
+        // rows of the other colour carry their value over
+        int iq, jq, kq;
+        cell_ijk(d, G.cell[s], iq, jq, kq);
+        if (((iq + jq + kq) & 1) != only) {
+            next[s] = prev ? prev[s] : T(0);
+            return;
+        }
+    }
     T a = 0;
 #pragma unroll
     for (int l = 0; l < 8; l++) {
@@ -781,12 +901,33 @@ void launch_rbgs_color(hipStream_t st, const Dims &d, const SysCoef<T> &S, const
 }
 
 template <typename T>
-int launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x, int prev) {
+int launch_rbgs_color_ghost(hipStream_t st, const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
+                            T *x, int color, const GhostRows<T> &G, int prev) {
+    if (prev < 0 || G.ng <= 0) abort();  // needs relaxed ghost values (not a zero start)
+    const int out = prev == 0 ? 1 : 0;
+    if constexpr (std::is_same<T, float>::value) {
+        if (d.nx >= kQuadMinRow) {
+            const unsigned gx = ((d.nx + 3) / 4 + 63) / 64, gy = (d.ny + 7) / 8;
+            const int gz = (int)((G.ng + (long)gx * gy * 64 - 1) / ((long)gx * gy * 64));
+            hipLaunchKernelGGL((k_rbgs_quad_g<8>), dim3(gx, gy, d.nz + gz), dim3(64, 8), 0, st, d, S, flag, b, x,
+                               color, G, S.is_p, (const float *)G.vals[prev], G.vals[out], gz);
+            return out;
+        }
+    }
+    const unsigned gx = ((d.nx + 1) / 2 + 63) / 64, gy = (d.ny + 3) / 4;
+    const int gz = (int)((G.ng + (long)gx * gy * 64 - 1) / ((long)gx * gy * 64));
+    hipLaunchKernelGGL((k_rbgs_g<T>), dim3(gx, gy, d.nz + gz), dim3(64, 4), 0, st, d, S, flag, b, x, color, G,
+                       S.is_p, (const T *)G.vals[prev], G.vals[out], gz);
+    return out;
+}
+
+template <typename T>
+int launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x, int prev, Dims d, int only) {
     const int out = prev == 0 ? 1 : 0;
     if (G.ng <= 0) return out;
     const int gb = (G.ng + 255) / 256;
     hipLaunchKernelGGL((k_ghost_relax<T>), dim3(gb), dim3(256), 0, st, G, is_p, b, x,
-                       prev < 0 ? (const T *)nullptr : (const T *)G.vals[prev], G.vals[out]);
+                       prev < 0 ? (const T *)nullptr : (const T *)G.vals[prev], G.vals[out], d, only);
     return out;
 }
 
@@ -1556,7 +1697,9 @@ void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, 
     template void launch_resid_restrict<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *,         \
                                            const T *, const T *, const Dims &, const MGTransfer &, const double *, \
                                            const double *, T *, const SysCoef<T> *, const uint8_t *, T *);        \
-    template int launch_ghost_relax<T>(hipStream_t, const GhostRows<T> &, int, const T *, T *, int);               \
+    template int launch_ghost_relax<T>(hipStream_t, const GhostRows<T> &, int, const T *, T *, int, Dims, int);    \
+    template int launch_rbgs_color_ghost<T>(hipStream_t, const Dims &, const SysCoef<T> &, const uint8_t *,        \
+                                            const T *, T *, int, const GhostRows<T> &, int);                      \
     template void launch_apply<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,             \
                                   const uint8_t *, const T *, T *, int, const T *, double *);                      \
     template void launch_residual<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,          \

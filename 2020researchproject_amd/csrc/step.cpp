@@ -568,12 +568,21 @@ static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const Ghost
                 // first_done: the kernel that wrote b already wrote it
                 const bool z = zero_init && sw == 0 && c == 0;
                 snprintf(nm, sizeof nm, "rbgs_L%d", level);
+                // Ghost rows are relaxed after each colour, those of the
+                // colour just updated (the others keep their values); past a
+                // zero start that happens inside the colour launch itself.
+                if (G.ng && !z && s->ghost_prev >= 0) {
+                    TIMED(nm, n * (2 * sizeof(T) + 1),
+                          s->ghost_prev = launch_rbgs_color_ghost<T>(s->stream, d, S, flag, b, x, c, G, s->ghost_prev));
+                    continue;
+                }
                 if (!(z && first_done))
                     TIMED(nm, z ? n * (1.5 * sizeof(T) + 1) : n * (2 * sizeof(T) + 1),
                           launch_rbgs_color<T>(s->stream, d, S, flag, b, x, c, z ? 1 : 0));
                 if (!G.ng) continue;
                 if (z) s->ghost_prev = -1;
-                TIMED("ghost_relax", 0, s->ghost_prev = launch_ghost_relax<T>(s->stream, G, S.is_p, b, x, s->ghost_prev));
+                TIMED("ghost_relax", 0,
+                      s->ghost_prev = launch_ghost_relax<T>(s->stream, G, S.is_p, b, x, s->ghost_prev, d, c));
             }
         return;
     }
