@@ -199,7 +199,7 @@ struct _ibm_solver {
     double mg_gamma = 3.5;       // coarsening threshold (multiples of the level's smallest width)
     int mg_pre = 2, mg_post = 2; // smoothing sweeps per level
     int mg_smoother = 1;         // 0 fused red-black sweeps, 1 one launch per colour
-    int mg_light_from = 0;       // levels >= this (if > 0) smooth with one sweep before and after
+    int mg_light_from = 1;       // levels >= this (if > 0) smooth with one sweep before and after
 
     // ---- host mirrors ----
     std::vector<double> lvset_h;     // reference layout (k fastest)

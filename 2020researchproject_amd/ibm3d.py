@@ -228,7 +228,7 @@ class Solver:
     def set_mg_params(self, gamma: float = 3.5, pre: int = 2, post: int = 2, smoother: int = 1):
         self.lib.IBMSolver_set_mg_params(self.h, gamma, pre, post, smoother)
 
-    def set_mg_light_levels(self, from_level: int = 0):
+    def set_mg_light_levels(self, from_level: int = 1):
         if hasattr(self.lib, "IBMSolver_set_mg_light_levels"):
             self.lib.IBMSolver_set_mg_light_levels(self.h, int(from_level))
 

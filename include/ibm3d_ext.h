@@ -58,9 +58,10 @@ void IBMSolver_set_precision(IBMSolver *, int mode);
    IBMSolver_assemble. */
 void IBMSolver_set_mg_params(IBMSolver *, double gamma, int pre, int post, int smoother);
 /* Levels from `from_level` down (when from_level > 0) smooth with one sweep
-   before and one after the coarse correction instead of pre/post; 0 (the
-   default) smooths every level with pre/post. Cheaper cycles, but the final
-   divergence at a given tolerance is larger. Before IBMSolver_assemble. */
+   before and one after the coarse correction instead of pre/post; 0 smooths
+   every level with pre/post. Default 1: the finest level takes pre/post
+   sweeps, every coarser level one (cheaper cycles at the same iteration
+   count on the sphere and cavity cases). Before IBMSolver_assemble. */
 void IBMSolver_set_mg_light_levels(IBMSolver *, int from_level);
 
 /* Copies a field to host memory in the reference layout (see above). Solid
