@@ -27,14 +27,18 @@ namespace ibm {
 // reads are then fetched by one XCD and reused from its L2, whatever the
 // number of blocks per plane.
 constexpr unsigned kNumXcd = 8;
-__device__ inline uint3 xcd_block(int on) {
-    if (!on) return make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
-    const unsigned gx = gridDim.x, gy = gridDim.y;
-    const unsigned n = gx * gy * gridDim.z;
-    const unsigned w = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+// Logical block (x fastest, then y, then z, on a gx x gy x - grid of n
+// blocks) of the block with linear id w, where XCD w % 8 runs it.
+__device__ inline uint3 xcd_map(unsigned w, unsigned n, unsigned gx, unsigned gy) {
     const unsigned x = w % kNumXcd, s = w / kNumXcd, q = n / kNumXcd, r = n % kNumXcd;
     const unsigned L = x * q + (x < r ? x : r) + s;
     return make_uint3(L % gx, (L / gx) % gy, L / (gx * gy));
+}
+__device__ inline uint3 xcd_block(int on) {
+    if (!on) return make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned w = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    return xcd_map(w, gx * gy * gridDim.z, gx, gy);
 }
 
 __device__ inline long blin() {
@@ -691,21 +695,22 @@ __device__ inline void ghost_row_in_colour(const Dims &d, const SysCoef<T> &S, c
     next[s] = v;
 }
 
-// one colour (not a zero start) with that colour's ghost rows: the first gz
-// planes of blocks relax ghost rows (64 per block), the others run the
-// one-colour sweep
+// One colour (not a zero start) with that colour's ghost rows, on a 1-D grid:
+// the first ngb blocks (a multiple of 8, dispatched first and spread over
+// the XCDs) relax ghost rows, one wave of 64 rows per block so the rows'
+// gathers spread over many CUs; the other blocks run the one-colour sweep on
+// a gx x gy x nz block grid, XCD-contiguous.
 template <typename T>
 __global__ void __launch_bounds__(256) k_rbgs_g(Dims d, SysCoef<T> S, const uint8_t *flag, const T *b, T *x,
-                                                int color, GhostRows<T> G, int is_p, const T *prev, T *next, int gz) {
-    uint3 bk = xcd_block(1);
-    if ((int)bk.z < gz) {
-        // one wave of rows per block: the rows' gathers spread over many CUs
-        const int s = (int)(((long)bk.z * gridDim.y + bk.y) * gridDim.x + bk.x) * 64 + threadIdx.x;
+                                                int color, GhostRows<T> G, int is_p, const T *prev, T *next,
+                                                unsigned ngb, unsigned gx, unsigned gy) {
+    const unsigned w = blockIdx.x;
+    if (w < ngb) {
+        const int s = (int)w * 64 + threadIdx.x;
         if (threadIdx.y == 0 && s < G.ng) ghost_row_in_colour(d, S, flag, b, x, G, is_p, prev, next, color, s);
         return;
     }
-    bk.z -= gz;
-    rbgs_cell(d, S, flag, b, x, color, 0, bk);
+    rbgs_cell(d, S, flag, b, x, color, 0, xcd_map(w - ngb, gridDim.x - ngb, gx, gy));
 }
 
 // four cells per lane (rbgs_quad.h) on levels with rows of at least this
@@ -718,19 +723,19 @@ __global__ void __launch_bounds__(64 * BY) k_rbgs_quad(Dims d, SysCoef<float> S,
     rbgs_quad<BY>(d, S, flag, b, x, color, zero_in, xcd_block(1));
 }
 
+// k_rbgs_g for the four-cells-per-lane sweep
 template <int BY>
 __global__ void __launch_bounds__(64 * BY) k_rbgs_quad_g(Dims d, SysCoef<float> S, const uint8_t *flag,
                                                          const float *b, float *x, int color, GhostRows<float> G,
-                                                         int is_p, const float *prev, float *next, int gz) {
-    uint3 bk = xcd_block(1);
-    if ((int)bk.z < gz) {
-        // one wave of rows per block: the rows' gathers spread over many CUs
-        const int s = (int)(((long)bk.z * gridDim.y + bk.y) * gridDim.x + bk.x) * 64 + threadIdx.x;
+                                                         int is_p, const float *prev, float *next, unsigned ngb,
+                                                         unsigned gx, unsigned gy) {
+    const unsigned w = blockIdx.x;
+    if (w < ngb) {
+        const int s = (int)w * 64 + threadIdx.x;
         if (threadIdx.y == 0 && s < G.ng) ghost_row_in_colour(d, S, flag, b, x, G, is_p, prev, next, color, s);
         return;
     }
-    bk.z -= gz;
-    rbgs_quad<BY>(d, S, flag, b, x, color, 0, bk);
+    rbgs_quad<BY>(d, S, flag, b, x, color, 0, xcd_map(w - ngb, gridDim.x - ngb, gx, gy));
 }
 
 // r = b - A x on the fluid rows (pinned row: b - x); no norms
@@ -905,19 +910,20 @@ int launch_rbgs_color_ghost(hipStream_t st, const Dims &d, const SysCoef<T> &S, 
                             T *x, int color, const GhostRows<T> &G, int prev) {
     if (prev < 0 || G.ng <= 0) abort();  // needs relaxed ghost values (not a zero start)
     const int out = prev == 0 ? 1 : 0;
+    // ghost blocks: 64 rows each, rounded up to a multiple of the XCD count
+    // so that the sweep blocks keep their XCD assignment
+    const unsigned ngb = (unsigned)(((G.ng + 63) / 64 + 7) / 8 * 8);
     if constexpr (std::is_same<T, float>::value) {
         if (d.nx >= kQuadMinRow) {
             const unsigned gx = ((d.nx + 3) / 4 + 63) / 64, gy = (d.ny + 7) / 8;
-            const int gz = (int)((G.ng + (long)gx * gy * 64 - 1) / ((long)gx * gy * 64));
-            hipLaunchKernelGGL((k_rbgs_quad_g<8>), dim3(gx, gy, d.nz + gz), dim3(64, 8), 0, st, d, S, flag, b, x,
-                               color, G, S.is_p, (const float *)G.vals[prev], G.vals[out], gz);
+            hipLaunchKernelGGL((k_rbgs_quad_g<8>), dim3(ngb + gx * gy * d.nz), dim3(64, 8), 0, st, d, S, flag, b, x,
+                               color, G, S.is_p, (const float *)G.vals[prev], G.vals[out], ngb, gx, gy);
             return out;
         }
     }
     const unsigned gx = ((d.nx + 1) / 2 + 63) / 64, gy = (d.ny + 3) / 4;
-    const int gz = (int)((G.ng + (long)gx * gy * 64 - 1) / ((long)gx * gy * 64));
-    hipLaunchKernelGGL((k_rbgs_g<T>), dim3(gx, gy, d.nz + gz), dim3(64, 4), 0, st, d, S, flag, b, x, color, G,
-                       S.is_p, (const T *)G.vals[prev], G.vals[out], gz);
+    hipLaunchKernelGGL((k_rbgs_g<T>), dim3(ngb + gx * gy * d.nz), dim3(64, 4), 0, st, d, S, flag, b, x, color, G,
+                       S.is_p, (const T *)G.vals[prev], G.vals[out], ngb, gx, gy);
     return out;
 }
 
