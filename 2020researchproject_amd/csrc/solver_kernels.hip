@@ -213,124 +213,137 @@ __device__ inline T first_colour(const SysCoef<T> &S, unsigned f, int i, int j, 
 // The kernels below compute the stencil of every cell before looking at its
 // flag (solid and ghost cells have in-bounds neighbours too; their results
 // are discarded), so all loads of a thread issue before the first wait.
-// TX: storage type of x; xo, when not null, receives x in type T (the
-// operand of a later vector update).
-template <typename T, int MODE, typename TX = T>
-__global__ void __launch_bounds__(256) k_apply(Dims d, SysCoef<T> S, const uint8_t *__restrict__ flag,
-                                               const TX *__restrict__ x, T *__restrict__ y, const T *__restrict__ w,
-                                               double *partial, T *__restrict__ xo) {
+// Ghost rows run inside the launches of the operator, residual and Jacobi
+// kernels: the first blocks of a 1-D grid (a multiple of 8, so they spread
+// over the XCDs and the other blocks keep their XCD assignment) each take 64
+// rows in their first wave; the other blocks do the fluid, solid and pinned
+// rows and leave the ghost cells' outputs to them.
+static inline unsigned ghost_blocks(int ng) { return ng > 0 ? (unsigned)(((ng + 63) / 64 + 7) / 8 * 8) : 0u; }
+
+// stores of a cell pair that skip ghost cells (g0, g1)
+template <typename T>
+__device__ inline void store_pair_skip(const Dims &d, int i0, long q0, T *a, T v0, T v1, bool g0, bool g1) {
     using T2 = typename Vec2<T>::type;
-    const int i0 = 2 * (blockIdx.x * 64 + threadIdx.x), j = blockIdx.y * 4 + threadIdx.y;
-    const int k0 = blockIdx.z * kRedZP, k1 = min(k0 + kRedZP, d.nz);
+    if (i0 + 1 < d.nx && !g0 && !g1) {
+        T2 o;
+        o.x = v0;
+        o.y = v1;
+        *reinterpret_cast<T2 *>(a + q0) = o;
+    } else {
+        if (!g0) a[q0] = v0;
+        if (i0 + 1 < d.nx && !g1) a[q0 + 1] = v1;
+    }
+}
+
+// y at ghost row s: is_p ? (1 - ws) x - sum w x : (1 + ws) x + sum w x
+template <typename T, typename TX>
+__device__ inline T ghost_row_apply(const GhostRows<T> &G, int is_p, const TX *x, int s, long &c) {
+    T a = 0;
+#pragma unroll
+    for (int l = 0; l < 8; l++) {
+        const long q = G.src[8 * (long)s + l];
+        if (q >= 0) a += G.w[8 * (long)s + l] * (T)x[q];
+    }
+    c = G.cell[s];
+    const T ws = G.wself[s], xc = (T)x[c];
+    return is_p ? (T(1) - ws) * xc - a : (T(1) + ws) * xc + a;
+}
+
+// y = A x; TX: storage type of x; xo, when not null, receives x in type T
+// (the operand of a later vector update).
+template <typename T, int MODE, typename TX = T>
+__global__ void __launch_bounds__(256) k_apply(Dims d, SysCoef<T> S, GhostRows<T> G, const uint8_t *__restrict__ flag,
+                                               const TX *__restrict__ x, T *__restrict__ y, const T *__restrict__ w,
+                                               double *partial, T *__restrict__ xo, unsigned ngb, unsigned gx,
+                                               unsigned gy) {
+    using T2 = typename Vec2<T>::type;
+    const unsigned wb = blockIdx.x;
     double acc[2] = {0, 0};
-    if (i0 < d.nx && j < d.ny) {
-        T yv[kRedZP][2], xq[kRedZP][2];
-        T2 wv[kRedZP];
-        unsigned fv[kRedZP];
-#pragma unroll
-        for (int kk = 0; kk < kRedZP; kk++) {
-            const int k = min(k0 + kk, k1 - 1);  // past the last plane: a repeat, discarded
-            const long q0 = cidx(d, i0, j, k);
-            fv[kk] = *reinterpret_cast<const unsigned short *>(flag + q0);
-            if (MODE > 0) wv[kk] = *reinterpret_cast<const T2 *>(w + q0);
-            pair_apply(d, S, x, i0, j, k, q0, yv[kk], xq[kk]);
+    if (wb < ngb) {
+        const int s = (int)wb * 64 + threadIdx.x;
+        if (threadIdx.y == 0 && s < G.ng) {
+            long c;
+            const T yv = ghost_row_apply<T>(G, S.is_p, x, s, c);
+            y[c] = yv;
+            if (MODE == 1) acc[0] = (double)w[c] * (double)yv;
+            if (MODE == 2) { acc[0] = (double)yv * (double)w[c]; acc[1] = (double)yv * (double)yv; }
         }
+    } else {
+        const uint3 bk = xcd_map(wb - ngb, gridDim.x - ngb, gx, gy);
+        const int i0 = 2 * (bk.x * 64 + threadIdx.x), j = bk.y * 4 + threadIdx.y;
+        const int k0 = bk.z * kRedZP, k1 = min(k0 + kRedZP, d.nz);
+        if (i0 < d.nx && j < d.ny) {
+            T yv[kRedZP][2], xq[kRedZP][2];
+            T2 wv[kRedZP];
+            unsigned fv[kRedZP];
 #pragma unroll
-        for (int kk = 0; kk < kRedZP; kk++) {
-            if (k0 + kk >= k1) break;
-            const long q0 = cidx(d, i0, j, k0 + kk);
-            T v[2];
-            bool on[2];
-#pragma unroll
-            for (int o = 0; o < 2; o++) {
-                const unsigned f = (fv[kk] >> (8 * o)) & 0xFFu;
-                const bool ident = q0 + o == S.pin || f == F_SOLID;
-                v[o] = ident ? xq[kk][o] : yv[kk][o];
-                on[o] = i0 + o < d.nx && (ident || f == F_FLUID);  // ghost rows: k_apply_ghost
+            for (int kk = 0; kk < kRedZP; kk++) {
+                const int k = min(k0 + kk, k1 - 1);  // past the last plane: a repeat, discarded
+                const long q0 = cidx(d, i0, j, k);
+                fv[kk] = *reinterpret_cast<const unsigned short *>(flag + q0);
+                if (MODE > 0) wv[kk] = *reinterpret_cast<const T2 *>(w + q0);
+                pair_apply(d, S, x, i0, j, k, q0, yv[kk], xq[kk]);
             }
-            // ghost cells are overwritten by k_apply_ghost afterwards
-            if (i0 + 1 < d.nx) {
-                T2 yy;
-                yy.x = v[0];
-                yy.y = v[1];
-                *reinterpret_cast<T2 *>(y + q0) = yy;
-                if (xo) {
-                    T2 xx;
-                    xx.x = xq[kk][0];
-                    xx.y = xq[kk][1];
-                    *reinterpret_cast<T2 *>(xo + q0) = xx;
+#pragma unroll
+            for (int kk = 0; kk < kRedZP; kk++) {
+                if (k0 + kk >= k1) break;
+                const long q0 = cidx(d, i0, j, k0 + kk);
+                T v[2];
+                bool on[2], gh[2];
+#pragma unroll
+                for (int o = 0; o < 2; o++) {
+                    const unsigned f = (fv[kk] >> (8 * o)) & 0xFFu;
+                    const bool ident = q0 + o == S.pin || f == F_SOLID;
+                    v[o] = ident ? xq[kk][o] : yv[kk][o];
+                    gh[o] = !ident && f == F_GHOST;  // written by the ghost blocks
+                    on[o] = i0 + o < d.nx && (ident || f == F_FLUID);
                 }
-            } else {
-                y[q0] = v[0];
-                if (xo) xo[q0] = xq[kk][0];
-            }
+                store_pair_skip(d, i0, q0, y, v[0], v[1], gh[0], gh[1]);
+                if (xo) store_pair_skip(d, i0, q0, xo, xq[kk][0], xq[kk][1], false, false);
 #pragma unroll
-            for (int o = 0; o < 2; o++) {
-                if (!on[o]) continue;
-                const double wo = o ? (double)wv[kk].y : (double)wv[kk].x;
-                if (MODE == 1) acc[0] += wo * (double)v[o];
-                if (MODE == 2) { acc[0] += (double)v[o] * wo; acc[1] += (double)v[o] * (double)v[o]; }
+                for (int o = 0; o < 2; o++) {
+                    if (!on[o]) continue;
+                    const double wo = o ? (double)wv[kk].y : (double)wv[kk].x;
+                    if (MODE == 1) acc[0] += wo * (double)v[o];
+                    if (MODE == 2) { acc[0] += (double)v[o] * wo; acc[1] += (double)v[o] * (double)v[o]; }
+                }
             }
         }
     }
-    if constexpr (MODE > 0) bsum<MODE>(acc, partial, blin());
+    if constexpr (MODE > 0) bsum<MODE>(acc, partial, wb);
 }
 
-template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_apply_ghost(GhostRows<T> G, int is_p, const T *x, T *y, const T *w,
-                                                     double *partial, long blk0) {
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    double acc[2] = {0, 0};
-    if (s < G.ng) {
-        T a = 0;
-#pragma unroll
-        for (int l = 0; l < 8; l++) {
-            const long c = G.src[8 * (long)s + l];
-            if (c >= 0) a += G.w[8 * (long)s + l] * x[c];
-        }
-        const long c = G.cell[s];
-        const T ws = G.wself[s];
-        const T yv = is_p ? (T(1) - ws) * x[c] - a : (T(1) + ws) * x[c] + a;
-        y[c] = yv;
-        if (MODE == 1) acc[0] = (double)w[c] * (double)yv;
-        if (MODE == 2) { acc[0] = (double)yv * (double)w[c]; acc[1] = (double)yv * (double)yv; }
-    }
-    if constexpr (MODE > 0) bsum<MODE>(acc, partial, blk0 + blockIdx.x);
-}
-
-long apply_partials(const Dims &d, int ng) { return pblocks(d) + (ng + 255) / 256; }
+long apply_partials(const Dims &d, int ng) { return pblocks(d) + ghost_blocks(ng); }
+// 1-D grid of the operator / residual kernels: ghost blocks + pair blocks
+static inline unsigned agrid_blocks(const Dims &d, int ng) { return (unsigned)apply_partials(d, ng); }
+static inline unsigned pgx(const Dims &d) { return ((d.nx + 1) / 2 + 63) / 64; }
+static inline unsigned pgy(const Dims &d) { return (d.ny + 3) / 4; }
 
 template <typename T>
 void launch_apply(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
                   const T *x, T *y, int mode, const T *w, double *partial) {
-    const long nb = pblocks(d);
-    const int gb = (G.ng + 255) / 256;
-    if (mode == 0) {
-        hipLaunchKernelGGL((k_apply<T, 0>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial, (T *)nullptr);
-        if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 0>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
-    } else if (mode == 1) {
-        hipLaunchKernelGGL((k_apply<T, 1>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial, (T *)nullptr);
-        if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 1>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
-    } else {
-        hipLaunchKernelGGL((k_apply<T, 2>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x, y, w, partial, (T *)nullptr);
-        if (gb) hipLaunchKernelGGL((k_apply_ghost<T, 2>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
-    }
+    const unsigned ngb = ghost_blocks(G.ng), nb = agrid_blocks(d, G.ng);
+    if (mode == 0)
+        hipLaunchKernelGGL((k_apply<T, 0>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, x, y, w, partial, (T *)nullptr,
+                           ngb, pgx(d), pgy(d));
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_apply<T, 1>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, x, y, w, partial, (T *)nullptr,
+                           ngb, pgx(d), pgy(d));
+    else
+        hipLaunchKernelGGL((k_apply<T, 2>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, x, y, w, partial, (T *)nullptr,
+                           ngb, pgx(d), pgy(d));
 }
 
 void launch_apply_f32in(hipStream_t st, const Dims &d, const SysCoef<double> &S, const GhostRows<double> &G,
                         const uint8_t *flag, const float *x32, double *x, double *y, int mode, const double *w,
                         double *partial) {
-    const long nb = pblocks(d);
-    const int gb = (G.ng + 255) / 256;
-    if (mode == 1) {
-        hipLaunchKernelGGL((k_apply<double, 1, float>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x32, y, w, partial, x);
-        if (gb)
-            hipLaunchKernelGGL((k_apply_ghost<double, 1>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
-    } else {
-        hipLaunchKernelGGL((k_apply<double, 2, float>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, x32, y, w, partial, x);
-        if (gb)
-            hipLaunchKernelGGL((k_apply_ghost<double, 2>), dim3(gb), dim3(256), 0, st, G, S.is_p, x, y, w, partial, nb);
-    }
+    const unsigned ngb = ghost_blocks(G.ng), nb = agrid_blocks(d, G.ng);
+    if (mode == 1)
+        hipLaunchKernelGGL((k_apply<double, 1, float>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, x32, y, w, partial,
+                           x, ngb, pgx(d), pgy(d));
+    else
+        hipLaunchKernelGGL((k_apply<double, 2, float>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, x32, y, w, partial,
+                           x, ngb, pgx(d), pgy(d));
 }
 
 // ---------------------------------------------------------------------------
@@ -343,98 +356,81 @@ __device__ inline double nscale(const NormSpec &ns, const Dims &d, int i, int j,
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_residual(Dims d, SysCoef<T> S, const uint8_t *__restrict__ flag,
-                                                  const T *__restrict__ b, const T *__restrict__ x,
-                                                  T *__restrict__ r, NormSpec ns, double *partial) {
+__global__ void __launch_bounds__(256) k_residual(Dims d, SysCoef<T> S, GhostRows<T> G,
+                                                  const uint8_t *__restrict__ flag, const T *__restrict__ b,
+                                                  const T *__restrict__ x, T *__restrict__ r, NormSpec ns,
+                                                  double *partial, unsigned ngb, unsigned gx, unsigned gy) {
     using T2 = typename Vec2<T>::type;
-    const int i0 = 2 * (blockIdx.x * 64 + threadIdx.x), j = blockIdx.y * 4 + threadIdx.y;
-    const int k0 = blockIdx.z * kRedZP, k1 = min(k0 + kRedZP, d.nz);
+    const unsigned wb = blockIdx.x;
     double acc[2] = {0, 0};
-    if (i0 < d.nx && j < d.ny) {
-        T yv[kRedZP][2], xq[kRedZP][2];
-        T2 bv[kRedZP];
-        unsigned fv[kRedZP];
-#pragma unroll
-        for (int kk = 0; kk < kRedZP; kk++) {
-            const int k = min(k0 + kk, k1 - 1);  // past the last plane: a repeat, discarded
-            const long q0 = cidx(d, i0, j, k);
-            fv[kk] = *reinterpret_cast<const unsigned short *>(flag + q0);
-            bv[kk] = *reinterpret_cast<const T2 *>(b + q0);
-            pair_apply(d, S, x, i0, j, k, q0, yv[kk], xq[kk]);
+    if (wb < ngb) {
+        const int s = (int)wb * 64 + threadIdx.x;
+        if (threadIdx.y == 0 && s < G.ng) {
+            long c;
+            const T yv = ghost_row_apply<T>(G, S.is_p, x, s, c);
+            const T rv = b[c] - yv;
+            r[c] = rv;
+            acc[0] = acc[1] = (double)rv * (double)rv;
         }
+    } else {
+        const uint3 bk = xcd_map(wb - ngb, gridDim.x - ngb, gx, gy);
+        const int i0 = 2 * (bk.x * 64 + threadIdx.x), j = bk.y * 4 + threadIdx.y;
+        const int k0 = bk.z * kRedZP, k1 = min(k0 + kRedZP, d.nz);
+        if (i0 < d.nx && j < d.ny) {
+            T yv[kRedZP][2], xq[kRedZP][2];
+            T2 bv[kRedZP];
+            unsigned fv[kRedZP];
 #pragma unroll
-        for (int kk = 0; kk < kRedZP; kk++) {
-            const int k = k0 + kk;
-            if (k >= k1) break;
-            const long q0 = cidx(d, i0, j, k);
-            T v[2];
-            bool on[2];
-#pragma unroll
-            for (int o = 0; o < 2; o++) {
-                const unsigned f = (fv[kk] >> (8 * o)) & 0xFFu;
-                const bool ident = q0 + o == S.pin || f == F_SOLID;
-                const T bo = o ? bv[kk].y : bv[kk].x;
-                v[o] = ident ? bo - xq[kk][o] : bo - yv[kk][o];
-                on[o] = i0 + o < d.nx && (ident || f == F_FLUID);  // ghost rows: k_residual_ghost
-            }
-            // ghost cells are overwritten by k_residual_ghost afterwards
-            if (i0 + 1 < d.nx) {
-                T2 rr;
-                rr.x = v[0];
-                rr.y = v[1];
-                *reinterpret_cast<T2 *>(r + q0) = rr;
-            } else {
-                r[q0] = v[0];
+            for (int kk = 0; kk < kRedZP; kk++) {
+                const int k = min(k0 + kk, k1 - 1);  // past the last plane: a repeat, discarded
+                const long q0 = cidx(d, i0, j, k);
+                fv[kk] = *reinterpret_cast<const unsigned short *>(flag + q0);
+                bv[kk] = *reinterpret_cast<const T2 *>(b + q0);
+                pair_apply(d, S, x, i0, j, k, q0, yv[kk], xq[kk]);
             }
 #pragma unroll
-            for (int o = 0; o < 2; o++) {
-                if (!on[o]) continue;
-                const double rd = (double)v[o];
-                acc[0] += rd * rd;
-                const double rsc = rd * nscale(ns, d, i0 + o, j, k, q0 + o);
-                acc[1] += rsc * rsc;
+            for (int kk = 0; kk < kRedZP; kk++) {
+                const int k = k0 + kk;
+                if (k >= k1) break;
+                const long q0 = cidx(d, i0, j, k);
+                T v[2];
+                bool on[2], gh[2];
+#pragma unroll
+                for (int o = 0; o < 2; o++) {
+                    const unsigned f = (fv[kk] >> (8 * o)) & 0xFFu;
+                    const bool ident = q0 + o == S.pin || f == F_SOLID;
+                    const T bo = o ? bv[kk].y : bv[kk].x;
+                    v[o] = ident ? bo - xq[kk][o] : bo - yv[kk][o];
+                    gh[o] = !ident && f == F_GHOST;  // written by the ghost blocks
+                    on[o] = i0 + o < d.nx && (ident || f == F_FLUID);
+                }
+                store_pair_skip(d, i0, q0, r, v[0], v[1], gh[0], gh[1]);
+#pragma unroll
+                for (int o = 0; o < 2; o++) {
+                    if (!on[o]) continue;
+                    const double rd = (double)v[o];
+                    acc[0] += rd * rd;
+                    const double rsc = rd * nscale(ns, d, i0 + o, j, k, q0 + o);
+                    acc[1] += rsc * rsc;
+                }
             }
         }
     }
-    bsum<2>(acc, partial, blin());
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_residual_ghost(GhostRows<T> G, int is_p, const T *b, const T *x, T *r,
-                                                        double *partial, long blk0) {
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    double acc[2] = {0, 0};
-    if (s < G.ng) {
-        T a = 0;
-#pragma unroll
-        for (int l = 0; l < 8; l++) {
-            const long c = G.src[8 * (long)s + l];
-            if (c >= 0) a += G.w[8 * (long)s + l] * x[c];
-        }
-        const long c = G.cell[s];
-        const T ws = G.wself[s];
-        const T yv = is_p ? (T(1) - ws) * x[c] - a : (T(1) + ws) * x[c] + a;
-        const T rv = b[c] - yv;
-        r[c] = rv;
-        acc[0] = acc[1] = (double)rv * (double)rv;
-    }
-    bsum<2>(acc, partial, blk0 + blockIdx.x);
+    bsum<2>(acc, partial, wb);
 }
 
 template <typename T>
 void launch_residual(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G,
                      const uint8_t *flag, const T *b, const T *x, T *r, const NormSpec &ns, double *partial) {
-    const long nb = pblocks(d);
-    hipLaunchKernelGGL((k_residual<T>), pgrid(d), dim3(64, 4), 0, st, d, S, flag, b, x, r, ns, partial);
-    const int gb = (G.ng + 255) / 256;
-    if (gb) hipLaunchKernelGGL((k_residual_ghost<T>), dim3(gb), dim3(256), 0, st, G, S.is_p, b, x, r, partial, nb);
+    hipLaunchKernelGGL((k_residual<T>), dim3(agrid_blocks(d, G.ng)), dim3(64, 4), 0, st, d, S, G, flag, b, x, r, ns,
+                       partial, ghost_blocks(G.ng), pgx(d), pgy(d));
 }
 
 // ---------------------------------------------------------------------------
 // Jacobi preconditioner z = D^-1 r
 // ---------------------------------------------------------------------------
 // Two cells of a row per lane. Fluid rows: r / diag; solid and pinned rows:
-// r (identity); ghost rows: written afterwards by k_jacobi_ghost.
+// r (identity); ghost rows: done by the ghost blocks of the launch.
 template <typename T>
 __device__ inline T jacobi_row(const Dims &d, const SysCoef<T> &S, unsigned f, int i, int j, int k, long q, T rv) {
     const T diag = stencil_diag(S, min(i, d.nx - 1), j, k, q);
@@ -442,41 +438,63 @@ __device__ inline T jacobi_row(const Dims &d, const SysCoef<T> &S, unsigned f, i
     return f == F_FLUID && diag != T(0) ? rv / diag : T(0);
 }
 
+// diagonal of ghost row s
 template <typename T>
-__device__ inline void store_pair(const Dims &d, int i0, long q0, T *a, T v0, T v1) {
-    using T2 = typename Vec2<T>::type;
-    if (i0 + 1 < d.nx) {
-        T2 o;
-        o.x = v0;
-        o.y = v1;
-        *reinterpret_cast<T2 *>(a + q0) = o;
-    } else {
-        a[q0] = v0;
-    }
+__device__ inline T ghost_diag(const GhostRows<T> &G, int is_p, int s) {
+    return is_p ? T(1) - G.wself[s] : T(1) + G.wself[s];
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_jacobi(Dims d, SysCoef<T> S, const uint8_t *flag, const T *r, T *z) {
+__global__ void __launch_bounds__(256) k_jacobi(Dims d, SysCoef<T> S, GhostRows<T> G, const uint8_t *flag,
+                                                const T *r, T *z, unsigned ngb, unsigned gx, unsigned gy) {
     using T2 = typename Vec2<T>::type;
-    const uint3 bk = xcd_block(1);
+    const unsigned wb = blockIdx.x;
+    if (wb < ngb) {
+        const int s = (int)wb * 64 + threadIdx.x;
+        if (threadIdx.y == 0 && s < G.ng) {
+            const long c = G.cell[s];
+            const T dg = ghost_diag(G, S.is_p, s);
+            z[c] = dg != T(0) ? r[c] / dg : r[c];
+        }
+        return;
+    }
+    const uint3 bk = xcd_map(wb - ngb, gridDim.x - ngb, gx, gy);
     const int i0 = 2 * (bk.x * 64 + threadIdx.x), j = bk.y * 4 + threadIdx.y, k = bk.z;
     if (i0 >= d.nx || j >= d.ny) return;
     const long q0 = cidx(d, i0, j, k);
     const unsigned fv = *reinterpret_cast<const unsigned short *>(flag + q0);
     const T2 rv = *reinterpret_cast<const T2 *>(r + q0);
-    store_pair(d, i0, q0, z, jacobi_row(d, S, fv & 0xFFu, i0, j, k, q0, rv.x),
-               jacobi_row(d, S, fv >> 8, i0 + 1, j, k, q0 + 1, rv.y));
+    const unsigned f0 = fv & 0xFFu, f1 = fv >> 8;
+    store_pair_skip(d, i0, q0, z, jacobi_row(d, S, f0, i0, j, k, q0, rv.x),
+                    jacobi_row(d, S, f1, i0 + 1, j, k, q0 + 1, rv.y), f0 == F_GHOST && q0 != S.pin,
+                    f1 == F_GHOST && q0 + 1 != S.pin);
 }
 
 // BiCGSTAB direction updates fused with the Jacobi preconditioner:
 //   SJ = 0: p = r + beta (p - omega v), ph = D^-1 p
 //   SJ = 1: s = r - alpha v,            sh = D^-1 s
-// Same arithmetic as k_bicg_p / k_bicg_s followed by k_jacobi.
+// Same arithmetic as k_bicg_p / k_bicg_s followed by k_jacobi; ghost cells
+// (both outputs) are done by the ghost blocks.
 template <typename T, int SJ>
-__global__ void __launch_bounds__(256) k_bicg_jacobi(Dims d, SysCoef<T> S, const uint8_t *flag, const KScal *ks,
-                                                     const T *r, const T *v, T *p, T *ph) {
+__global__ void __launch_bounds__(256) k_bicg_jacobi(Dims d, SysCoef<T> S, GhostRows<T> G, const uint8_t *flag,
+                                                     const KScal *ks, const T *r, const T *v, T *p, T *ph,
+                                                     unsigned ngb, unsigned gx, unsigned gy) {
     using T2 = typename Vec2<T>::type;
-    const uint3 bk = xcd_block(1);
+    const unsigned wb = blockIdx.x;
+    if (wb < ngb) {
+        const int s = (int)wb * 64 + threadIdx.x;
+        if (threadIdx.y == 0 && s < G.ng) {
+            const long c = G.cell[s];
+            T n;
+            if (SJ == 0) n = r[c] + (T)ks->beta * (p[c] - (T)ks->omega * v[c]);
+            else n = r[c] - (T)ks->alpha * v[c];
+            p[c] = n;
+            const T dg = ghost_diag(G, S.is_p, s);
+            ph[c] = dg != T(0) ? n / dg : n;
+        }
+        return;
+    }
+    const uint3 bk = xcd_map(wb - ngb, gridDim.x - ngb, gx, gy);
     const int i0 = 2 * (bk.x * 64 + threadIdx.x), j = bk.y * 4 + threadIdx.y, k = bk.z;
     if (i0 >= d.nx || j >= d.ny) return;
     const long q0 = cidx(d, i0, j, k);
@@ -494,29 +512,23 @@ __global__ void __launch_bounds__(256) k_bicg_jacobi(Dims d, SysCoef<T> S, const
         n0 = rv.x - alpha * vv.x;
         n1 = rv.y - alpha * vv.y;
     }
-    store_pair(d, i0, q0, p, n0, n1);
-    store_pair(d, i0, q0, ph, jacobi_row(d, S, fv & 0xFFu, i0, j, k, q0, n0),
-               jacobi_row(d, S, fv >> 8, i0 + 1, j, k, q0 + 1, n1));
-}
-
-template <typename T>
-__global__ void k_jacobi_ghost(GhostRows<T> G, int is_p, const T *r, T *z) {
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= G.ng) return;
-    const long c = G.cell[s];
-    const T dg = is_p ? T(1) - G.wself[s] : T(1) + G.wself[s];
-    z[c] = dg != T(0) ? r[c] / dg : r[c];
+    const unsigned f0 = fv & 0xFFu, f1 = fv >> 8;
+    const bool g0 = f0 == F_GHOST && q0 != S.pin, g1 = f1 == F_GHOST && q0 + 1 != S.pin;
+    store_pair_skip(d, i0, q0, p, n0, n1, g0, g1);
+    store_pair_skip(d, i0, q0, ph, jacobi_row(d, S, f0, i0, j, k, q0, n0), jacobi_row(d, S, f1, i0 + 1, j, k, q0 + 1, n1),
+                    g0, g1);
 }
 
 // one cell pair per lane, one plane per block
 static inline dim3 jgrid(const Dims &d) { return dim3(((d.nx + 1) / 2 + 63) / 64, (d.ny + 3) / 4, d.nz); }
+// the same blocks on a 1-D grid behind the ghost blocks
+static inline unsigned jblocks(const Dims &d, int ng) { return ghost_blocks(ng) + pgx(d) * pgy(d) * d.nz; }
 
 template <typename T>
 void launch_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G, const uint8_t *flag,
                    const T *r, T *z) {
-    hipLaunchKernelGGL((k_jacobi<T>), jgrid(d), dim3(64, 4), 0, st, d, S, flag, r, z);
-    const int gb = (G.ng + 255) / 256;
-    if (gb) hipLaunchKernelGGL((k_jacobi_ghost<T>), dim3(gb), dim3(256), 0, st, G, S.is_p, r, z);
+    hipLaunchKernelGGL((k_jacobi<T>), dim3(jblocks(d, G.ng)), dim3(64, 4), 0, st, d, S, G, flag, r, z,
+                       ghost_blocks(G.ng), pgx(d), pgy(d));
 }
 
 // BiCGSTAB direction update in FP64 (SJ 0: p = r + beta (p - omega v); SJ 1:
@@ -573,12 +585,13 @@ void launch_bicg_d2f(hipStream_t st, const Dims &d, const SysCoef<float> *S, con
 template <typename T>
 void launch_bicg_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G,
                         const uint8_t *flag, const KScal *ks, int which, const T *r, const T *v, T *p, T *ph) {
+    const unsigned ngb = ghost_blocks(G.ng), nb = jblocks(d, G.ng);
     if (which == 0)
-        hipLaunchKernelGGL((k_bicg_jacobi<T, 0>), jgrid(d), dim3(64, 4), 0, st, d, S, flag, ks, r, v, p, ph);
+        hipLaunchKernelGGL((k_bicg_jacobi<T, 0>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, ks, r, v, p, ph, ngb,
+                           pgx(d), pgy(d));
     else
-        hipLaunchKernelGGL((k_bicg_jacobi<T, 1>), jgrid(d), dim3(64, 4), 0, st, d, S, flag, ks, r, v, p, ph);
-    const int gb = (G.ng + 255) / 256;
-    if (gb) hipLaunchKernelGGL((k_jacobi_ghost<T>), dim3(gb), dim3(256), 0, st, G, S.is_p, p, ph);
+        hipLaunchKernelGGL((k_bicg_jacobi<T, 1>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, ks, r, v, p, ph, ngb,
+                           pgx(d), pgy(d));
 }
 
 // ---------------------------------------------------------------------------
