@@ -96,7 +96,10 @@ int launch_rbgs_color_ghost(hipStream_t st, const Dims &d, const SysCoef<T> &S, 
 
 // BiCGSTAB pieces
 template <typename T>
-void launch_bicg_init(hipStream_t st, const Dims &d, T *r, T *rh, T *p, T *v, T *x, const T *b);
+// x = 0, r = rh = b, p = v = 0; partial (vec_partials blocks x 2): the
+// norms of r = b, ||r||^2 and ||N r||^2, as launch_residual gives for x = 0
+void launch_bicg_init(hipStream_t st, const Dims &d, T *r, T *rh, T *p, T *v, T *x, const T *b, const NormSpec &ns,
+                      double *partial);
 template <typename T>
 void launch_bicg_restart(hipStream_t st, const Dims &d, const T *r, T *rh, T *p, T *v);
 template <typename T>

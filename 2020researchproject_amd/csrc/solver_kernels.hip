@@ -954,12 +954,24 @@ int launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T 
 // BiCGSTAB vector kernels (van der Vorst 1992) with device-resident scalars.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_bicg_init(Dims d, T *r, T *rh, T *p, T *v, T *x, const T *b) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-    if (i >= d.nx || j >= d.ny) return;
-    const long q = cidx(d, i, j, k);
-    const T bv = b[q];
-    r[q] = bv; rh[q] = bv; p[q] = 0; v[q] = 0; x[q] = 0;
+__global__ void __launch_bounds__(256) k_bicg_init(Dims d, T *r, T *rh, T *p, T *v, T *x, const T *b, NormSpec ns,
+                                                   double *partial) {
+    // x0 = 0, so r0 = b on every row (fluid, solid, pinned and ghost): the
+    // norms of r0 are those of b (||b||^2, ||N b||^2), kRedZ planes per block
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * kRedZ, k1 = min(k0 + kRedZ, d.nz);
+    double acc[2] = {0, 0};
+    if (i < d.nx && j < d.ny) {
+        for (int k = k0; k < k1; k++) {
+            const long q = cidx(d, i, j, k);
+            const T bv = b[q];
+            r[q] = bv; rh[q] = bv; p[q] = 0; v[q] = 0; x[q] = 0;
+            const double bd = (double)bv, bs = bd * nscale(ns, d, i, j, k, q);
+            acc[0] += bd * bd;
+            acc[1] += bs * bs;
+        }
+    }
+    bsum<2>(acc, partial, blin());
 }
 
 // restart from the current iterate: r holds b - A x; the shadow residual
@@ -1018,8 +1030,9 @@ __global__ void __launch_bounds__(256) k_bicg_xr(Dims d, const KScal *ks, const 
 long vec_partials(const Dims &d) { return iblocks(d); }
 
 template <typename T>
-void launch_bicg_init(hipStream_t st, const Dims &d, T *r, T *rh, T *p, T *v, T *x, const T *b) {
-    hipLaunchKernelGGL((k_bicg_init<T>), igrid(d), dim3(64, 4), 0, st, d, r, rh, p, v, x, b);
+void launch_bicg_init(hipStream_t st, const Dims &d, T *r, T *rh, T *p, T *v, T *x, const T *b, const NormSpec &ns,
+                      double *partial) {
+    hipLaunchKernelGGL((k_bicg_init<T>), rgrid(d), dim3(64, 4), 0, st, d, r, rh, p, v, x, b, ns, partial);
 }
 template <typename T>
 void launch_bicg_restart(hipStream_t st, const Dims &d, const T *r, T *rh, T *p, T *v) {
@@ -1726,7 +1739,8 @@ void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, 
     template void launch_jacobi<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,            \
                                    const uint8_t *, const T *, T *);                                               \
     template void launch_bicg_restart<T>(hipStream_t, const Dims &, const T *, T *, T *, T *);                \
-    template void launch_bicg_init<T>(hipStream_t, const Dims &, T *, T *, T *, T *, T *, const T *);              \
+    template void launch_bicg_init<T>(hipStream_t, const Dims &, T *, T *, T *, T *, T *, const T *,               \
+                                      const NormSpec &, double *);                                                \
     template void launch_bicg_p<T>(hipStream_t, const Dims &, const KScal *, const T *, const T *, T *);           \
     template void launch_bicg_s<T>(hipStream_t, const Dims &, const KScal *, const T *, const T *, T *);           \
     template void launch_bicg_jacobi<T>(hipStream_t, const Dims &, const SysCoef<T> &, const GhostRows<T> &,       \

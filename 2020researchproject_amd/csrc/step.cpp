@@ -747,10 +747,9 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
     const double cb = ncells(s) * sizeof(T);
     char nm[64];
 
-    launch_bicg_init<T>(s->stream, d, r, rh, p, v, x, b);
-    // norms of b (= r0) through the residual kernel with x = 0
-    TIMED("residual", 3 * cb, launch_residual<T>(s->stream, d, S, G, flag, b, x, r, ns, s->partial.p));
-    finalize_read(s, 3, na, 2);
+    // x0 = 0: r0 = b, whose norms the initialisation kernel forms
+    TIMED("bicg_init", 6 * cb, launch_bicg_init<T>(s->stream, d, r, rh, p, v, x, b, ns, s->partial.p));
+    finalize_read(s, 3, nv, 2);
     const double bn = bnorm > 0 ? bnorm : std::sqrt(ns.on ? s->ks_h->bbn : s->ks_h->bb);
     if (bn == 0) { *final_rel = 0; return 0; }
     double rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
