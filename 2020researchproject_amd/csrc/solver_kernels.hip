@@ -197,13 +197,18 @@ __device__ inline T stencil_diag(const SysCoef<T> &S, int i, int j, int k, long 
 // i + j + k even), the value the one-colour smoother writes at (i, j, k):
 // same arithmetic (the neighbour sum is 0), so a kernel that produces a
 // level's right-hand side can also write this colour and save that launch.
+// first_colour_of: the same from a diagonal formed beforehand (its loads can
+// then issue ahead of those that produce bv).
 template <typename T>
-__device__ inline T first_colour(const SysCoef<T> &S, unsigned f, int i, int j, int k, long q, T bv) {
+__device__ inline T first_colour_of(const SysCoef<T> &S, unsigned f, int i, int j, int k, long q, T bv, T diag) {
     if ((i + j + k) & 1) return T(0);
     if (q == S.pin || f == F_SOLID) return bv;
     if (f != F_FLUID) return T(0);
-    const T diag = stencil_diag(S, i, j, k, q);
     return diag != T(0) ? (bv + T(0)) / diag : T(0);
+}
+template <typename T>
+__device__ inline T first_colour(const SysCoef<T> &S, unsigned f, int i, int j, int k, long q, T bv) {
+    return first_colour_of(S, f, i, j, k, q, bv, stencil_diag(S, i, j, k, q));
 }
 
 // ---------------------------------------------------------------------------
@@ -545,6 +550,9 @@ __global__ void __launch_bounds__(256) k_bicg_d2f(Dims d, SysCoef<float> S, cons
     const double2 rv = *reinterpret_cast<const double2 *>(r + q0);
     const double2 vv = *reinterpret_cast<const double2 *>(v + q0);
     const unsigned fv = FIRST ? *reinterpret_cast<const unsigned short *>(flag + q0) : 0u;
+    // diagonals of the pair (loaded ahead of the update)
+    const float dg0 = FIRST ? stencil_diag(S, i0, j, k, q0) : 0.f;
+    const float dg1 = FIRST ? stencil_diag(S, min(i0 + 1, d.nx - 1), j, k, q0 + 1) : 0.f;
     double n0, n1;
     if (SJ == 0) {
         const double2 pv = *reinterpret_cast<const double2 *>(p + q0);
@@ -561,12 +569,13 @@ __global__ void __launch_bounds__(256) k_bicg_d2f(Dims d, SysCoef<float> S, cons
         *reinterpret_cast<double2 *>(p + q0) = make_double2(n0, n1);
         *reinterpret_cast<float2 *>(b32 + q0) = make_float2(b0, b1);
         if (FIRST)
-            *reinterpret_cast<float2 *>(x32 + q0) = make_float2(first_colour(S, fv & 0xFFu, i0, j, k, q0, b0),
-                                                                first_colour(S, fv >> 8, i0 + 1, j, k, q0 + 1, b1));
+            *reinterpret_cast<float2 *>(x32 + q0) =
+                make_float2(first_colour_of(S, fv & 0xFFu, i0, j, k, q0, b0, dg0),
+                            first_colour_of(S, fv >> 8, i0 + 1, j, k, q0 + 1, b1, dg1));
     } else {
         p[q0] = n0;
         b32[q0] = b0;
-        if (FIRST) x32[q0] = first_colour(S, fv & 0xFFu, i0, j, k, q0, b0);
+        if (FIRST) x32[q0] = first_colour_of(S, fv & 0xFFu, i0, j, k, q0, b0, dg0);
     }
 }
 
@@ -809,6 +818,10 @@ __global__ void __launch_bounds__(256) k_resid_restrict(Dims df, SysCoef<T> S, c
     const double vx[2] = {C1(dxf, ii[0]), ni > 1 ? C1(dxf, ii[1]) : 0.0};
     const double vy[2] = {C1(dyf, jj[0]), nj > 1 ? C1(dyf, jj[1]) : 0.0};
     const double vz[2] = {C1(dzf, kk[0]), nk > 1 ? C1(dzf, kk[1]) : 0.0};
+    // the coarse cell's diagonal and flag (first colour), loaded ahead
+    const long qc = cidx(dc, I, J, K);
+    const T cdiag = FIRST ? stencil_diag(Sc, I, J, K, qc) : T(0);
+    const uint8_t cflag = FIRST ? flag_c[qc] : (uint8_t)0;
     T rv[8];
     uint8_t fv[8];
 #pragma unroll
@@ -825,10 +838,9 @@ __global__ void __launch_bounds__(256) k_resid_restrict(Dims df, SysCoef<T> S, c
 #pragma unroll
     for (int c = 0; c < 8; c++)
         if (fv[c] == F_FLUID) acc += (double)rv[c] * vx[c & 1] * vy[(c >> 1) & 1] * vz[c >> 2];
-    const long qc = cidx(dc, I, J, K);
     const T bv = (T)(acc / (C1(dxc, I) * C1(dyc, J) * C1(dzc, K)));
     bc[qc] = bv;
-    if (FIRST) xc[qc] = first_colour(Sc, flag_c[qc], I, J, K, qc, bv);
+    if (FIRST) xc[qc] = first_colour_of(Sc, cflag, I, J, K, qc, bv, cdiag);
 }
 
 template <typename T>
@@ -1341,10 +1353,11 @@ __global__ void __launch_bounds__(256) k_d2f_first(Dims d, SysCoef<float> S, con
     const long q0 = cidx(d, i0, j, k);
     const double2 av = *reinterpret_cast<const double2 *>(a + q0);
     const unsigned fv = *reinterpret_cast<const unsigned short *>(flag + q0);
+    const float dg0 = stencil_diag(S, i0, j, k, q0), dg1 = stencil_diag(S, min(i0 + 1, d.nx - 1), j, k, q0 + 1);
     const float b0 = (float)av.x, b1 = (float)av.y;
-    const float x0 = first_colour(S, fv & 0xFFu, i0, j, k, q0, b0);
+    const float x0 = first_colour_of(S, fv & 0xFFu, i0, j, k, q0, b0, dg0);
     if (i0 + 1 < d.nx) {
-        const float x1 = first_colour(S, fv >> 8, i0 + 1, j, k, q0 + 1, b1);
+        const float x1 = first_colour_of(S, fv >> 8, i0 + 1, j, k, q0 + 1, b1, dg1);
         *reinterpret_cast<float2 *>(b + q0) = make_float2(b0, b1);
         *reinterpret_cast<float2 *>(x + q0) = make_float2(x0, x1);
     } else {
@@ -1424,9 +1437,17 @@ __global__ void __launch_bounds__(256) k_restrict(Dims df, Dims dc, MGTransfer t
     T rv[kRestZ][8];
     uint8_t fv[kRestZ][8];
     double vz[kRestZ][2];
+    T cdiag[kRestZ];
+    uint8_t cflag[kRestZ];
 #pragma unroll
     for (int z = 0; z < kRestZ; z++) {
         const int K = min(K0 + z, dc.nz - 1);  // past the last plane: a repeat, not written
+        if (FIRST) {
+            // the coarse cell's diagonal and flag, loaded ahead of the children
+            const long qc = cidx(dc, I, J, K);
+            cdiag[z] = stencil_diag(Sc, I, J, K, qc);
+            cflag[z] = flag_c[qc];
+        }
         const int k0 = tr.sk[K], nk = tr.sk[K + 1] - k0;
         const int kk[2] = {k0, k0 + nk - 1};
         vz[z][0] = C1(dzf, kk[0]);
@@ -1449,7 +1470,7 @@ __global__ void __launch_bounds__(256) k_restrict(Dims df, Dims dc, MGTransfer t
         const long qc = cidx(dc, I, J, K);
         const T bv = (T)(acc / (C1(dxc, I) * C1(dyc, J) * C1(dzc, K)));
         bc[qc] = bv;
-        if (FIRST) xc[qc] = first_colour(Sc, flag_c[qc], I, J, K, qc, bv);
+        if (FIRST) xc[qc] = first_colour_of(Sc, cflag[z], I, J, K, qc, bv, cdiag[z]);
     }
 }
 
