@@ -53,8 +53,8 @@ IBMSolver *IBMSolver_new(const int num_process, const int rank, const int Px, co
     DevBytesScope account(&s->dev_bytes);
     HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_CHECK(hipHostMalloc((void **)&s->ks_h, sizeof(KScal), hipHostMallocDefault));
-    HIP_CHECK(hipHostMalloc((void **)&s->pub_h, sizeof(IBMSolver::KPub), hipHostMallocCoherent | hipHostMallocMapped));
-    memset((void *)s->pub_h, 0, sizeof(IBMSolver::KPub));
+    HIP_CHECK(hipHostMalloc((void **)&s->pub_h, sizeof(KPub), hipHostMallocCoherent | hipHostMallocMapped));
+    memset((void *)s->pub_h, 0, sizeof(KPub));
     HIP_CHECK(hipHostGetDevicePointer((void **)&s->pub_d, s->pub_h, 0));
     HIP_CHECK(hipHostMalloc((void **)&s->sums_h, kSumSlots * sizeof(double), hipHostMallocDefault));
     HIP_CHECK(hipHostMalloc((void **)&s->gm_h, 64 * sizeof(double), hipHostMallocDefault));
@@ -73,6 +73,12 @@ void IBMSolver_destroy(IBMSolver *s) {
     for (auto &g : s->vgraphs) (void)hipGraphExecDestroy(g.exec);
     (void)hipHostFree(s->ks_h);
     (void)hipHostFree(s->pub_h);
+    for (auto &L : s->mlane) {
+        if (L.pub_h) (void)hipHostFree(L.pub_h);
+        if (L.done) (void)hipEventDestroy(L.done);
+        if (L.st) (void)hipStreamDestroy(L.st);
+    }
+    if (s->ev_rhs) (void)hipEventDestroy(s->ev_rhs);
     (void)hipHostFree(s->sums_h);
     (void)hipHostFree(s->gm_h);
     (void)hipStreamDestroy(s->stream);
@@ -562,6 +568,23 @@ static void alloc_fields(IBMSolver *s) {
     if (s->turb_type != TURBMODEL_NONE)
         for (int a = 0; a < 6; a++) s->tau[a].alloc(n);
     for (int v = 0; v < 8; v++) s->kry.v64[v].alloc(n);
+    if (s->precision != IBM3D_PREC_MIXED) {
+        // lanes of the concurrent momentum solves (components 1 and 2)
+        HipGlobalLock lk(hip_global_mutex());
+        const long np = std::max({conv_rhs_blocks(s->d), apply_partials(s->d, s->ng), vec_partials(s->d)}) + 64;
+        for (auto &L : s->mlane) {
+            for (int v = 0; v < 8; v++) L.v[v].alloc(n);
+            L.ks.alloc(1);
+            L.partial.alloc((size_t)np * 4);
+            HIP_CHECK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+            HIP_CHECK(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+            HIP_CHECK(hipHostMalloc((void **)&L.pub_h, sizeof(KPub), hipHostMallocCoherent | hipHostMallocMapped));
+            memset((void *)L.pub_h, 0, sizeof(KPub));
+            HIP_CHECK(hipHostGetDevicePointer((void **)&L.pub_d, L.pub_h, 0));
+        }
+        HIP_CHECK(hipEventCreateWithFlags(&s->ev_rhs, hipEventDisableTiming));
+        s->mom_lanes = true;
+    }
     if (s->precision != IBM3D_PREC_FP64) {
         if (s->precision == IBM3D_PREC_MIXED)
             for (int v = 0; v < 8; v++) s->kry.v32[v].alloc(n);

@@ -164,6 +164,27 @@ struct GmresData {
     DBuf<double> dev, partial;  // dev: 64 scalars
 };
 
+// Krylov scalars published by the finalize kernel into device-mapped,
+// coherent host memory with an epoch counter (the host waits on it).
+struct KPub {
+    KScal ks;
+    unsigned epoch;
+    unsigned pad;
+};
+
+// One lane of the concurrent momentum solves (components 1 and 2; component
+// 0 uses the solver's own stream and buffers): a stream with its own Krylov
+// vectors, device scalars, block partials and publication slot.
+struct MomLane {
+    hipStream_t st = nullptr;
+    DBuf<double> v[8];
+    DBuf<KScal> ks;
+    DBuf<double> partial;
+    KPub *pub_h = nullptr, *pub_d = nullptr;
+    unsigned epoch = 0;
+    hipEvent_t done = nullptr;  // recorded on st after the lane's last kernel
+};
+
 }  // namespace ibm
 
 struct _ibm_solver {
@@ -224,16 +245,13 @@ struct _ibm_solver {
     ibm::DBuf<double> partial, sums;  // sums: kSumSlots doubles
     ibm::DBuf<ibm::KScal> ks;
     ibm::KScal *ks_h = nullptr;      // pinned
-    // Krylov scalars published by the finalize kernel into device-mapped,
-    // coherent host memory with an epoch counter (the host waits on it)
-    struct KPub {
-        ibm::KScal ks;
-        unsigned epoch;
-        unsigned pad;
-    };
-    KPub *pub_h = nullptr;           // host address
-    KPub *pub_d = nullptr;           // device address of the same memory
+    ibm::KPub *pub_h = nullptr;      // host address (see KPub)
+    ibm::KPub *pub_d = nullptr;      // device address of the same memory
     unsigned pub_epoch = 0;
+    // concurrent momentum solves (IBM3D_PREC_FP64 and _MIXED_KRYLOV)
+    ibm::MomLane mlane[2];
+    hipEvent_t ev_rhs = nullptr;     // momentum right-hand sides formed (solver's stream)
+    bool mom_lanes = false;
     double *sums_h = nullptr;        // pinned
     double *gm_h = nullptr;          // pinned, 64 scalars of the GMRES / PCG solvers
 

@@ -189,19 +189,14 @@ static void read_sums(IBMSolver *s, int nv) {
     HIP_CHECK(hipStreamSynchronize(s->stream));
 }
 
-// Finalize kernel `op` over the block partials, with the resulting scalars
-// published into host memory, then a wait for them there: no copy and no
-// stream synchronisation on the Krylov loop's path (each end of iteration
-// otherwise costs a copy and a wake-up of the host thread). The stream is
-// polled now and then so that an error or a lost publication cannot make
-// the wait endless.
-static void finalize_read(IBMSolver *s, int op, long nblk, int nv) {
-    const unsigned e = ++s->pub_epoch;
-    launch_finalize(s->stream, op, s->partial.p, nblk, nv, s->ks.p, &s->pub_d->ks, &s->pub_d->epoch, e);
-    unsigned *ep = &s->pub_h->epoch;
+// Waits until the finalize kernel enqueued on `st` has published epoch e in
+// pub_h, polling the stream now and then so that an error or a lost
+// publication cannot make the wait endless.
+static void wait_published(hipStream_t st, KPub *pub_h, unsigned e) {
+    unsigned *ep = &pub_h->epoch;
     for (long n = 1; __atomic_load_n(ep, __ATOMIC_ACQUIRE) != e; n++) {
         if ((n & 1023) == 0) {
-            const hipError_t q = hipStreamQuery(s->stream);
+            const hipError_t q = hipStreamQuery(st);
             if (q == hipSuccess) {
                 if (__atomic_load_n(ep, __ATOMIC_ACQUIRE) == e) break;
                 fprintf(stderr, "error: Krylov scalars were not published (epoch %u, expected %u)\n",
@@ -212,6 +207,16 @@ static void finalize_read(IBMSolver *s, int op, long nblk, int nv) {
         }
         __builtin_ia32_pause();
     }
+}
+
+// Finalize kernel `op` over the block partials, with the resulting scalars
+// published into host memory, then a wait for them there: no copy and no
+// stream synchronisation on the Krylov loop's path (each end of iteration
+// otherwise costs a copy and a wake-up of the host thread).
+static void finalize_read(IBMSolver *s, int op, long nblk, int nv) {
+    const unsigned e = ++s->pub_epoch;
+    launch_finalize(s->stream, op, s->partial.p, nblk, nv, s->ks.p, &s->pub_d->ks, &s->pub_d->epoch, e);
+    wait_published(s->stream, s->pub_h, e);
     memcpy((void *)s->ks_h, (const void *)&s->pub_h->ks, sizeof(KScal));
 }
 
@@ -1081,6 +1086,134 @@ static int refine_mixed(IBMSolver *s, int sk, const double *b, double *x, int pr
 // BiCGSTAB iterations between restarts when the preconditioner runs in FP32
 static const int kMixedRestart = 20;
 
+// Records the outcome of the solve of system sk and reacts like the reference
+// (src/ibm3d_fracstep.c:363-369, 673-679): a non-finite residual aborts,
+// one above the tolerance warns.
+static void report_solve(IBMSolver *s, int sk, double rel, int it) {
+    s->final_norm[sk] = rel;
+    s->final_iter[sk] = it;
+    if (!std::isfinite(rel)) {
+        if (sk < 3) fprintf(stderr, "error: floating pointer error raised in u%d_star\n", sk + 1);
+        else fprintf(stderr, "error: floating pointer error raised in p_prime\n");
+        abort();
+    }
+    if (rel > s->tol) {
+        if (sk < 3) fprintf(stderr, "warning: u%d_star did not converge\n", sk + 1);
+        else fprintf(stderr, "warning: p_prime did not converge\n");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The three momentum solves at once (IBM3D_PREC_FP64 and _MIXED_KRYLOV):
+// BiCGSTAB with Jacobi preconditioning from x0 = 0, component 0 on the
+// solver's stream and buffers, components 1 and 2 on lanes of their own.
+// Each iteration is enqueued for every active component before the host
+// waits for any of their scalars, so the components' kernels, reductions and
+// host reads overlap on the device. Per component: the kernels and decisions
+// of bicgstab<double> with prec 1 and no restarts.
+// ---------------------------------------------------------------------------
+static void momentum_concurrent(IBMSolver *s, const double bnorm[3]) {
+    const Dims &d = s->d;
+    const GhostRows<double> &G = s->gmom.G64;
+    const uint8_t *flag = s->flag.p;
+    const long nv = vec_partials(d), na = apply_partials(d, G.ng);
+    const NormSpec ns{0, nullptr, nullptr, nullptr, nullptr, -1};
+    const int maxit = 200;  // src/ibm3d_setup.c:1413
+    struct Lane {
+        hipStream_t st;
+        DBuf<double> *V;
+        KScal *ks;
+        double *partial;
+        KPub *pub_h, *pub_d;
+        unsigned *epoch;
+    };
+    Lane ln[3] = {{s->stream, s->kry.v64, s->ks.p, s->partial.p, s->pub_h, s->pub_d, &s->pub_epoch}, {}, {}};
+    for (int c = 1; c < 3; c++) {
+        MomLane &L = s->mlane[c - 1];
+        ln[c] = {L.st, L.v, L.ks.p, L.partial.p, L.pub_h, L.pub_d, &L.epoch};
+    }
+    // the right-hand sides were formed on the solver's stream
+    HIP_CHECK(hipEventRecord(s->ev_rhs, s->stream));
+    for (int c = 1; c < 3; c++) HIP_CHECK(hipStreamWaitEvent(ln[c].st, s->ev_rhs, 0));
+    KScal kh[3];
+    unsigned ep[3];
+    auto publish = [&](int c, int op, long nblk, int nvals) {
+        ep[c] = ++*ln[c].epoch;
+        launch_finalize(ln[c].st, op, ln[c].partial, nblk, nvals, ln[c].ks, &ln[c].pub_d->ks, &ln[c].pub_d->epoch,
+                        ep[c]);
+    };
+    auto collect = [&](int c) {
+        wait_published(ln[c].st, ln[c].pub_h, ep[c]);
+        memcpy((void *)&kh[c], (const void *)&ln[c].pub_h->ks, sizeof(KScal));
+    };
+    double bn[3], rn[3];
+    int it[3] = {0, 0, 0};
+    bool act[3];
+    for (int c = 0; c < 3; c++) {
+        DBuf<double> *V = ln[c].V;
+        launch_bicg_init<double>(ln[c].st, d, V[0].p, V[1].p, V[2].p, V[3].p, s->us[c].p, s->b[c].p, ns,
+                                 ln[c].partial);
+        publish(c, 3, nv, 2);
+    }
+    for (int c = 0; c < 3; c++) {
+        collect(c);
+        bn[c] = bnorm[c] > 0 ? bnorm[c] : std::sqrt(kh[c].bb);
+        rn[c] = std::sqrt(kh[c].rr);
+        act[c] = bn[c] != 0;
+    }
+    const bool zero[3] = {!act[0], !act[1], !act[2]};
+    for (;;) {
+        bool any = false;
+        for (int c = 0; c < 3; c++) {
+            act[c] = act[c] && it[c] < maxit && rn[c] > s->tol * bn[c];
+            any = any || act[c];
+        }
+        if (!any) break;
+        for (int c = 0; c < 3; c++) {
+            if (!act[c]) continue;
+            it[c]++;
+            const SysCoef<double> &S = s->sys[c].S64;
+            DBuf<double> *V = ln[c].V;
+            double *r = V[0].p, *rh = V[1].p, *p = V[2].p, *v = V[3].p, *ph = V[4].p, *sv = V[5].p, *sh = V[6].p,
+                   *t = V[7].p;
+            const hipStream_t st = ln[c].st;
+            launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, 0, r, v, p, ph);
+            launch_apply<double>(st, d, S, G, flag, ph, v, 1, rh, ln[c].partial);
+            launch_finalize(st, 0, ln[c].partial, na, 1, ln[c].ks);
+            launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, 1, r, v, sv, sh);
+            launch_apply<double>(st, d, S, G, flag, sh, t, 2, sv, ln[c].partial);
+            launch_finalize(st, 1, ln[c].partial, na, 2, ln[c].ks);
+            launch_bicg_xr<double>(st, d, ln[c].ks, ph, sh, sv, t, rh, s->us[c].p, r, ns, ln[c].partial);
+            publish(c, 2, nv, 3);
+        }
+        for (int c = 0; c < 3; c++) {
+            if (!act[c]) continue;
+            collect(c);
+            rn[c] = std::sqrt(kh[c].rr);
+            if (kh[c].breakdown || !std::isfinite(rn[c])) act[c] = false;
+        }
+    }
+    // true residuals of the final iterates
+    for (int c = 0; c < 3; c++) {
+        if (zero[c]) continue;
+        launch_residual<double>(ln[c].st, d, s->sys[c].S64, G, flag, s->b[c].p, s->us[c].p, ln[c].V[0].p, ns,
+                                ln[c].partial);
+        publish(c, 4, na, 2);
+    }
+    double rel[3] = {0, 0, 0};
+    for (int c = 0; c < 3; c++) {
+        if (zero[c]) continue;
+        collect(c);
+        rel[c] = std::sqrt(kh[c].rr) / bn[c];
+    }
+    // the solver's stream continues after the lanes' work
+    for (int c = 1; c < 3; c++) {
+        HIP_CHECK(hipEventRecord(s->mlane[c - 1].done, ln[c].st));
+        HIP_CHECK(hipStreamWaitEvent(s->stream, s->mlane[c - 1].done, 0));
+    }
+    for (int c = 0; c < 3; c++) report_solve(s, c, rel[c], it[c]);
+}
+
 static void solve_system(IBMSolver *s, int sk, const double *b, double *x, double bnorm) {
     const int prec = sk == 3 ? (s->precond_type == PRECOND_AMG ? 2 : 1) : 1;
     NormSpec ns{0, nullptr, nullptr, nullptr, nullptr, -1};
@@ -1107,17 +1240,7 @@ static void solve_system(IBMSolver *s, int sk, const double *b, double *x, doubl
         it = bicgstab<double>(s, s->sys[sk].S64, GD.G64, b, x, prec, ns, s->tol, maxit, bnorm, &rel,
                               sk == 3 ? "p64" : "u64");
     }
-    s->final_norm[sk] = rel;
-    s->final_iter[sk] = it;
-    if (!std::isfinite(rel)) {
-        if (sk < 3) fprintf(stderr, "error: floating pointer error raised in u%d_star\n", sk + 1);
-        else fprintf(stderr, "error: floating pointer error raised in p_prime\n");
-        abort();  // src/ibm3d_fracstep.c:363-366, 673-676
-    }
-    if (rel > s->tol) {
-        if (sk < 3) fprintf(stderr, "warning: u%d_star did not converge\n", sk + 1);
-        else fprintf(stderr, "warning: p_prime did not converge\n");
-    }
+    report_solve(s, sk, rel, it);
 }
 
 // ---------------------------------------------------------------------------
@@ -1206,10 +1329,19 @@ static void do_step(IBMSolver *s) {
         for (int f = 0; conv && f < 6; f++)
             if (s->bc[f].type == BC_CONVECTIVE_OUTFLOW) bn2[l] += s->sums_h[kConvNormSlot + 3 * f + l];
     }
-    // momentum Helmholtz solves
-    for (int l = 0; l < 3; l++) {
-        solve_system(s, l, s->b[l].p, s->us[l].p, std::sqrt(bn2[l]));
-        TIMED("outer_sol", 0, launch_outer(s->stream, s->geo, ob, s->us[l].p, nullptr, nullptr, 2, l));
+    // momentum Helmholtz solves: the three at once on their lanes, or one
+    // after the other (mixed defect-correction mode, and under the per-kernel
+    // timers, whose events are recorded on the solver's stream)
+    if (s->mom_lanes && s->timing == 0 && s->precision != IBM3D_PREC_MIXED) {
+        const double bn[3] = {std::sqrt(bn2[0]), std::sqrt(bn2[1]), std::sqrt(bn2[2])};
+        momentum_concurrent(s, bn);
+        for (int l = 0; l < 3; l++)
+            TIMED("outer_sol", 0, launch_outer(s->stream, s->geo, ob, s->us[l].p, nullptr, nullptr, 2, l));
+    } else {
+        for (int l = 0; l < 3; l++) {
+            solve_system(s, l, s->b[l].p, s->us[l].p, std::sqrt(bn2[l]));
+            TIMED("outer_sol", 0, launch_outer(s->stream, s->geo, ob, s->us[l].p, nullptr, nullptr, 2, l));
+        }
     }
     // K2: U*, Poisson RHS
     {
