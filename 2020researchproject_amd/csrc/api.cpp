@@ -591,6 +591,7 @@ static void alloc_fields(IBMSolver *s) {
         s->ref_r64.alloc(n);
         s->ref_rhs32.alloc(n);
         s->ref_e32.alloc(n);
+        if (s->precision == IBM3D_PREC_MIXED_KRYLOV) s->ref_e32b.alloc(n);
     }
     if (s->solver_type != SOLVER_BiCGSTAB) {
         // pressure solver scratch: GMRES basis, or the PCG / stationary

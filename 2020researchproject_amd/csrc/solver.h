@@ -273,6 +273,7 @@ struct _ibm_solver {
     ibm::GmresData gm;                    // SOLVER_GMRES / SOLVER_PCG / SOLVER_AMG scratch (pressure)
     ibm::DBuf<double> ref_r64;            // mixed mode: FP64 defect
     ibm::DBuf<float> ref_rhs32, ref_e32;  // mixed mode: FP32 defect and correction
+    ibm::DBuf<float> ref_e32b;            // mixed-Krylov mode: the second cycle output of an iteration
 
     // diagnostics
     double final_norm[4] = {0, 0, 0, 0};

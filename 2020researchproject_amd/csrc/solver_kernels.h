@@ -63,7 +63,7 @@ void launch_bicg_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, cons
 void launch_bicg_d2f(hipStream_t st, const Dims &d, const SysCoef<float> *S, const uint8_t *flag, const KScal *ks,
                      int which, const double *r, const double *v, double *p, float *b32, float *x32);
 // y = A x for x stored in FP32 (x32): as launch_apply (modes 1 and 2) on x =
-// (double) x32, which is also written to x.
+// (double) x32, which is also written to x unless x is null.
 void launch_apply_f32in(hipStream_t st, const Dims &d, const SysCoef<double> &S, const GhostRows<double> &G,
                         const uint8_t *flag, const float *x32, double *x, double *y, int mode, const double *w,
                         double *partial);
@@ -110,6 +110,11 @@ template <typename T>
 void launch_bicg_xr(hipStream_t st, const Dims &d, const KScal *ks, const T *ph, const T *sh, const T *s,
                     const T *t, const T *rh, T *x, T *r, const NormSpec &ns, double *partial);
 long vec_partials(const Dims &d);
+// launch_bicg_xr<double> with the preconditioned directions stored in FP32
+// (the FP32 multigrid outputs; converted exactly on load)
+void launch_bicg_xr_f32dir(hipStream_t st, const Dims &d, const KScal *ks, const float *ph, const float *sh,
+                           const double *s, const double *t, const double *rh, double *x, double *r,
+                           const NormSpec &ns, double *partial);
 
 // Restarted GMRES, PCG and stationary multigrid: up to kVecSetMax vectors
 // per multi-dot / multi-update launch.

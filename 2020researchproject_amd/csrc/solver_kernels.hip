@@ -1013,8 +1013,9 @@ __global__ void __launch_bounds__(256) k_bicg_s(Dims d, const KScal *ks, const T
     s[q] = r[q] - (T)ks->alpha * v[q];
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_bicg_xr(Dims d, const KScal *ks, const T *ph, const T *sh, const T *s,
+// TP: storage type of the preconditioned directions ph, sh
+template <typename T, typename TP = T>
+__global__ void __launch_bounds__(256) k_bicg_xr(Dims d, const KScal *ks, const TP *ph, const TP *sh, const T *s,
                                                  const T *t, const T *rh, T *x, T *r, NormSpec ns, double *partial) {
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
     const int k0 = blockIdx.z * kRedZ, k1 = min(k0 + kRedZ, d.nz);
@@ -1026,7 +1027,7 @@ __global__ void __launch_bounds__(256) k_bicg_xr(Dims d, const KScal *ks, const 
             const int k = k0 + kk;
             if (k >= k1) break;
             const long q = cidx(d, i, j, k);
-            x[q] = x[q] + alpha * ph[q] + omega * sh[q];
+            x[q] = x[q] + alpha * (T)ph[q] + omega * (T)sh[q];
             const T rv = s[q] - omega * t[q];
             r[q] = rv;
             const double rd = (double)rv;
@@ -1062,6 +1063,12 @@ template <typename T>
 void launch_bicg_xr(hipStream_t st, const Dims &d, const KScal *ks, const T *ph, const T *sh, const T *s,
                     const T *t, const T *rh, T *x, T *r, const NormSpec &ns, double *partial) {
     hipLaunchKernelGGL((k_bicg_xr<T>), rgrid(d), dim3(64, 4), 0, st, d, ks, ph, sh, s, t, rh, x, r, ns, partial);
+}
+void launch_bicg_xr_f32dir(hipStream_t st, const Dims &d, const KScal *ks, const float *ph, const float *sh,
+                           const double *s, const double *t, const double *rh, double *x, double *r,
+                           const NormSpec &ns, double *partial) {
+    hipLaunchKernelGGL((k_bicg_xr<double, float>), rgrid(d), dim3(64, 4), 0, st, d, ks, ph, sh, s, t, rh, x, r, ns,
+                       partial);
 }
 
 // One block of kFinThreads threads sums the block partials: thread t adds

@@ -727,11 +727,10 @@ static void mg_precond_f32(IBMSolver *s, const double *in, double *out) {
     });
 }
 
-// The FP32 V-cycle on ref_rhs32 into ref_e32 alone (its conversions fused
+// The FP32 V-cycle on ref_rhs32 into `out` alone (its conversions fused
 // into the neighbouring Krylov kernels); first_done as for vcycle.
-static void mg_vcycle_f32(IBMSolver *s, bool first_done) {
-    replay_or_capture(s, s->ref_rhs32.p, s->ref_e32.p,
-                      [&]() { vcycle<float>(s, 0, s->ref_rhs32.p, s->ref_e32.p, first_done); });
+static void mg_vcycle_f32(IBMSolver *s, bool first_done, float *out) {
+    replay_or_capture(s, s->ref_rhs32.p, out, [&]() { vcycle<float>(s, 0, s->ref_rhs32.p, out, first_done); });
 }
 
 // ---------------------------------------------------------------------------
@@ -774,7 +773,9 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
     const bool f32mg = prec == 3 && std::is_same<T, double>::value;
     const bool fuse1 = f32mg && fuse_first_colour(s, 0);
     const SysCoef<float> *S32f = fuse1 ? &s->sys[3].S32 : nullptr;
-    float *x32f = fuse1 ? s->ref_e32.p : nullptr;
+    // the iteration's two FP32 cycle outputs (M^-1 p, M^-1 s): read by the
+    // operator and the update in FP32, no FP64 copies
+    float *e32a = f32mg ? s->ref_e32.p : nullptr, *e32b = f32mg ? s->ref_e32b.p : nullptr;
     int it = 0;
     for (;;) {
     while (it < maxit && rn > tol * bn) {
@@ -788,8 +789,8 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         } else if (f32mg) {
             if constexpr (std::is_same<T, double>::value) {
                 TIMED("bicg_d2f", (fuse1 ? 5.5 : 5) * cb, launch_bicg_d2f(s->stream, d, S32f, flag, s->ks.p, 0, r, v, p,
-                                                                     s->ref_rhs32.p, x32f));
-                mg_vcycle_f32(s, fuse1);
+                                                                     s->ref_rhs32.p, fuse1 ? e32a : nullptr));
+                mg_vcycle_f32(s, fuse1, e32a);
             }
         } else {
             TIMED("bicg_vec", 4 * cb, launch_bicg_p<T>(s->stream, d, s->ks.p, r, v, p));
@@ -797,7 +798,7 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         }
         if constexpr (std::is_same<T, double>::value) {
             if (f32mg)
-                TIMED(nm, 3.5 * cb, launch_apply_f32in(s->stream, d, S, G, flag, s->ref_e32.p, ph, v, 1, rh,
+                TIMED(nm, 2.5 * cb, launch_apply_f32in(s->stream, d, S, G, flag, e32a, nullptr, v, 1, rh,
                                                        s->partial.p));
             else
                 TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, ph, v, 1, rh, s->partial.p));
@@ -810,8 +811,8 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         } else if (f32mg) {
             if constexpr (std::is_same<T, double>::value) {
                 TIMED("bicg_d2f", (fuse1 ? 4.5 : 4) * cb, launch_bicg_d2f(s->stream, d, S32f, flag, s->ks.p, 1, r, v, sv,
-                                                                     s->ref_rhs32.p, x32f));
-                mg_vcycle_f32(s, fuse1);
+                                                                     s->ref_rhs32.p, fuse1 ? e32b : nullptr));
+                mg_vcycle_f32(s, fuse1, e32b);
             }
         } else {
             TIMED("bicg_vec", 3 * cb, launch_bicg_s<T>(s->stream, d, s->ks.p, r, v, sv));
@@ -819,7 +820,7 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         }
         if constexpr (std::is_same<T, double>::value) {
             if (f32mg)
-                TIMED(nm, 3.5 * cb, launch_apply_f32in(s->stream, d, S, G, flag, s->ref_e32.p, sh, t, 2, sv,
+                TIMED(nm, 2.5 * cb, launch_apply_f32in(s->stream, d, S, G, flag, e32b, nullptr, t, 2, sv,
                                                        s->partial.p));
             else
                 TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, sh, t, 2, sv, s->partial.p));
@@ -827,7 +828,16 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
             TIMED(nm, 3 * cb, launch_apply<T>(s->stream, d, S, G, flag, sh, t, 2, sv, s->partial.p));
         }
         launch_finalize(s->stream, 1, s->partial.p, na, 2, s->ks.p);
-        TIMED("bicg_xr", 8 * cb, launch_bicg_xr<T>(s->stream, d, s->ks.p, ph, sh, sv, t, rh, x, r, ns, s->partial.p));
+        if constexpr (std::is_same<T, double>::value) {
+            if (f32mg)
+                TIMED("bicg_xr", 7 * cb, launch_bicg_xr_f32dir(s->stream, d, s->ks.p, e32a, e32b, sv, t, rh, x, r, ns,
+                                                               s->partial.p));
+            else
+                TIMED("bicg_xr", 8 * cb, launch_bicg_xr<T>(s->stream, d, s->ks.p, ph, sh, sv, t, rh, x, r, ns,
+                                                           s->partial.p));
+        } else {
+            TIMED("bicg_xr", 8 * cb, launch_bicg_xr<T>(s->stream, d, s->ks.p, ph, sh, sv, t, rh, x, r, ns, s->partial.p));
+        }
         finalize_read(s, 2, nv, 3);
         rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
         const bool bad = s->ks_h->breakdown || !std::isfinite(rn);
