@@ -574,6 +574,8 @@ static void alloc_fields(IBMSolver *s) {
         const long np = std::max({conv_rhs_blocks(s->d), apply_partials(s->d, s->ng), vec_partials(s->d)}) + 64;
         for (auto &L : s->mlane) {
             for (int v = 0; v < 8; v++) L.v[v].alloc(n);
+            if (s->precision == IBM3D_PREC_MIXED_KRYLOV)
+                for (int v = 0; v < 2; v++) L.d32[v].alloc(n);
             L.ks.alloc(1);
             L.partial.alloc((size_t)np * 4);
             HIP_CHECK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));

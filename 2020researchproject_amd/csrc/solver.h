@@ -178,6 +178,7 @@ struct KPub {
 struct MomLane {
     hipStream_t st = nullptr;
     DBuf<double> v[8];
+    DBuf<float> d32[2];         // FP32 preconditioned directions (mixed-Krylov mode)
     DBuf<KScal> ks;
     DBuf<double> partial;
     KPub *pub_h = nullptr, *pub_d = nullptr;

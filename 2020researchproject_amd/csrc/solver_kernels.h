@@ -56,6 +56,13 @@ template <typename T>
 void launch_bicg_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G,
                         const uint8_t *flag, const KScal *ks, int which, const T *r, const T *v, T *p, T *ph);
 
+// launch_bicg_jacobi<double> with ph stored in FP32 (rounded once: the
+// operator and the update then use exactly these values, so the iteration's
+// residual recurrence stays that of the iterate)
+void launch_bicg_jacobi_f32(hipStream_t st, const Dims &d, const SysCoef<double> &S, const GhostRows<double> &G,
+                            const uint8_t *flag, const KScal *ks, int which, const double *r, const double *v,
+                            double *p, float *ph);
+
 // BiCGSTAB direction update in FP64 (which 0: p = r + beta (p - omega v);
 // 1: p = r - alpha v, the s vector) that also writes b32 = (float) p and,
 // when x32 is not null, the first smoother colour on b32 with system *S

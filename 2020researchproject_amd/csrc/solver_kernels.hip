@@ -479,10 +479,10 @@ __global__ void __launch_bounds__(256) k_jacobi(Dims d, SysCoef<T> S, GhostRows<
 //   SJ = 0: p = r + beta (p - omega v), ph = D^-1 p
 //   SJ = 1: s = r - alpha v,            sh = D^-1 s
 // Same arithmetic as k_bicg_p / k_bicg_s followed by k_jacobi; ghost cells
-// (both outputs) are done by the ghost blocks.
-template <typename T, int SJ>
+// (both outputs) are done by the ghost blocks. TPH: storage type of ph.
+template <typename T, int SJ, typename TPH = T>
 __global__ void __launch_bounds__(256) k_bicg_jacobi(Dims d, SysCoef<T> S, GhostRows<T> G, const uint8_t *flag,
-                                                     const KScal *ks, const T *r, const T *v, T *p, T *ph,
+                                                     const KScal *ks, const T *r, const T *v, T *p, TPH *ph,
                                                      unsigned ngb, unsigned gx, unsigned gy) {
     using T2 = typename Vec2<T>::type;
     const unsigned wb = blockIdx.x;
@@ -495,7 +495,7 @@ __global__ void __launch_bounds__(256) k_bicg_jacobi(Dims d, SysCoef<T> S, Ghost
             else n = r[c] - (T)ks->alpha * v[c];
             p[c] = n;
             const T dg = ghost_diag(G, S.is_p, s);
-            ph[c] = dg != T(0) ? n / dg : n;
+            ph[c] = (TPH)(dg != T(0) ? n / dg : n);
         }
         return;
     }
@@ -520,8 +520,8 @@ __global__ void __launch_bounds__(256) k_bicg_jacobi(Dims d, SysCoef<T> S, Ghost
     const unsigned f0 = fv & 0xFFu, f1 = fv >> 8;
     const bool g0 = f0 == F_GHOST && q0 != S.pin, g1 = f1 == F_GHOST && q0 + 1 != S.pin;
     store_pair_skip(d, i0, q0, p, n0, n1, g0, g1);
-    store_pair_skip(d, i0, q0, ph, jacobi_row(d, S, f0, i0, j, k, q0, n0), jacobi_row(d, S, f1, i0 + 1, j, k, q0 + 1, n1),
-                    g0, g1);
+    store_pair_skip(d, i0, q0, ph, (TPH)jacobi_row(d, S, f0, i0, j, k, q0, n0),
+                    (TPH)jacobi_row(d, S, f1, i0 + 1, j, k, q0 + 1, n1), g0, g1);
 }
 
 // one cell pair per lane, one plane per block
@@ -601,6 +601,18 @@ void launch_bicg_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, cons
     else
         hipLaunchKernelGGL((k_bicg_jacobi<T, 1>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, ks, r, v, p, ph, ngb,
                            pgx(d), pgy(d));
+}
+
+void launch_bicg_jacobi_f32(hipStream_t st, const Dims &d, const SysCoef<double> &S, const GhostRows<double> &G,
+                            const uint8_t *flag, const KScal *ks, int which, const double *r, const double *v,
+                            double *p, float *ph) {
+    const unsigned ngb = ghost_blocks(G.ng), nb = jblocks(d, G.ng);
+    if (which == 0)
+        hipLaunchKernelGGL((k_bicg_jacobi<double, 0, float>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, ks, r, v, p,
+                           ph, ngb, pgx(d), pgy(d));
+    else
+        hipLaunchKernelGGL((k_bicg_jacobi<double, 1, float>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, ks, r, v, p,
+                           ph, ngb, pgx(d), pgy(d));
 }
 
 // ---------------------------------------------------------------------------

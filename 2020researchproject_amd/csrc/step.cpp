@@ -773,9 +773,13 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
     const bool f32mg = prec == 3 && std::is_same<T, double>::value;
     const bool fuse1 = f32mg && fuse_first_colour(s, 0);
     const SysCoef<float> *S32f = fuse1 ? &s->sys[3].S32 : nullptr;
-    // the iteration's two FP32 cycle outputs (M^-1 p, M^-1 s): read by the
+    // Mixed-Krylov mode with Jacobi (momentum): the preconditioned directions
+    // are stored in FP32 as well (rounded once, then used unchanged by the
+    // operator and the update).
+    const bool f32mom = prec == 1 && std::is_same<T, double>::value && s->precision == IBM3D_PREC_MIXED_KRYLOV;
+    // the iteration's two FP32 directions (M^-1 p, M^-1 s): read by the
     // operator and the update in FP32, no FP64 copies
-    float *e32a = f32mg ? s->ref_e32.p : nullptr, *e32b = f32mg ? s->ref_e32b.p : nullptr;
+    float *e32a = f32mg || f32mom ? s->ref_e32.p : nullptr, *e32b = f32mg || f32mom ? s->ref_e32b.p : nullptr;
     int it = 0;
     for (;;) {
     while (it < maxit && rn > tol * bn) {
@@ -785,7 +789,15 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         // update writes the FP32 right-hand side (and the cycle's first
         // colour), the operator reads the FP32 result and writes ph.
         if (prec == 1) {
-            TIMED("bicg_jacobi", 5 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 0, r, v, p, ph));
+            if constexpr (std::is_same<T, double>::value) {
+                if (f32mom)
+                    TIMED("bicg_jacobi", 4.5 * cb, launch_bicg_jacobi_f32(s->stream, d, S, G, flag, s->ks.p, 0, r, v, p,
+                                                                        e32a));
+                else
+                    TIMED("bicg_jacobi", 5 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 0, r, v, p, ph));
+            } else {
+                TIMED("bicg_jacobi", 5 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 0, r, v, p, ph));
+            }
         } else if (f32mg) {
             if constexpr (std::is_same<T, double>::value) {
                 TIMED("bicg_d2f", (fuse1 ? 5.5 : 5) * cb, launch_bicg_d2f(s->stream, d, S32f, flag, s->ks.p, 0, r, v, p,
@@ -797,7 +809,7 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
             precond(p, ph);
         }
         if constexpr (std::is_same<T, double>::value) {
-            if (f32mg)
+            if (f32mg || f32mom)
                 TIMED(nm, 2.5 * cb, launch_apply_f32in(s->stream, d, S, G, flag, e32a, nullptr, v, 1, rh,
                                                        s->partial.p));
             else
@@ -807,7 +819,15 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         }
         launch_finalize(s->stream, 0, s->partial.p, na, 1, s->ks.p);
         if (prec == 1) {
-            TIMED("bicg_jacobi", 4 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 1, r, v, sv, sh));
+            if constexpr (std::is_same<T, double>::value) {
+                if (f32mom)
+                    TIMED("bicg_jacobi", 3.5 * cb, launch_bicg_jacobi_f32(s->stream, d, S, G, flag, s->ks.p, 1, r, v, sv,
+                                                                        e32b));
+                else
+                    TIMED("bicg_jacobi", 4 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 1, r, v, sv, sh));
+            } else {
+                TIMED("bicg_jacobi", 4 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 1, r, v, sv, sh));
+            }
         } else if (f32mg) {
             if constexpr (std::is_same<T, double>::value) {
                 TIMED("bicg_d2f", (fuse1 ? 4.5 : 4) * cb, launch_bicg_d2f(s->stream, d, S32f, flag, s->ks.p, 1, r, v, sv,
@@ -819,7 +839,7 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
             precond(sv, sh);
         }
         if constexpr (std::is_same<T, double>::value) {
-            if (f32mg)
+            if (f32mg || f32mom)
                 TIMED(nm, 2.5 * cb, launch_apply_f32in(s->stream, d, S, G, flag, e32b, nullptr, t, 2, sv,
                                                        s->partial.p));
             else
@@ -829,7 +849,7 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         }
         launch_finalize(s->stream, 1, s->partial.p, na, 2, s->ks.p);
         if constexpr (std::is_same<T, double>::value) {
-            if (f32mg)
+            if (f32mg || f32mom)
                 TIMED("bicg_xr", 7 * cb, launch_bicg_xr_f32dir(s->stream, d, s->ks.p, e32a, e32b, sv, t, rh, x, r, ns,
                                                                s->partial.p));
             else
@@ -1136,11 +1156,16 @@ static void momentum_concurrent(IBMSolver *s, const double bnorm[3]) {
         double *partial;
         KPub *pub_h, *pub_d;
         unsigned *epoch;
+        float *da, *db;  // FP32 preconditioned directions (mixed-Krylov mode)
     };
-    Lane ln[3] = {{s->stream, s->kry.v64, s->ks.p, s->partial.p, s->pub_h, s->pub_d, &s->pub_epoch}, {}, {}};
+    // mixed-Krylov mode: preconditioned directions stored in FP32 (as in bicgstab)
+    const bool f32 = s->precision == IBM3D_PREC_MIXED_KRYLOV;
+    Lane ln[3] = {{s->stream, s->kry.v64, s->ks.p, s->partial.p, s->pub_h, s->pub_d, &s->pub_epoch,
+                   f32 ? s->ref_e32.p : nullptr, f32 ? s->ref_e32b.p : nullptr}, {}, {}};
     for (int c = 1; c < 3; c++) {
         MomLane &L = s->mlane[c - 1];
-        ln[c] = {L.st, L.v, L.ks.p, L.partial.p, L.pub_h, L.pub_d, &L.epoch};
+        ln[c] = {L.st, L.v, L.ks.p, L.partial.p, L.pub_h, L.pub_d, &L.epoch, f32 ? L.d32[0].p : nullptr,
+                 f32 ? L.d32[1].p : nullptr};
     }
     // the right-hand sides were formed on the solver's stream
     HIP_CHECK(hipEventRecord(s->ev_rhs, s->stream));
@@ -1187,13 +1212,26 @@ static void momentum_concurrent(IBMSolver *s, const double bnorm[3]) {
             double *r = V[0].p, *rh = V[1].p, *p = V[2].p, *v = V[3].p, *ph = V[4].p, *sv = V[5].p, *sh = V[6].p,
                    *t = V[7].p;
             const hipStream_t st = ln[c].st;
-            launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, 0, r, v, p, ph);
-            launch_apply<double>(st, d, S, G, flag, ph, v, 1, rh, ln[c].partial);
+            if (f32) {
+                launch_bicg_jacobi_f32(st, d, S, G, flag, ln[c].ks, 0, r, v, p, ln[c].da);
+                launch_apply_f32in(st, d, S, G, flag, ln[c].da, nullptr, v, 1, rh, ln[c].partial);
+            } else {
+                launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, 0, r, v, p, ph);
+                launch_apply<double>(st, d, S, G, flag, ph, v, 1, rh, ln[c].partial);
+            }
             launch_finalize(st, 0, ln[c].partial, na, 1, ln[c].ks);
-            launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, 1, r, v, sv, sh);
-            launch_apply<double>(st, d, S, G, flag, sh, t, 2, sv, ln[c].partial);
+            if (f32) {
+                launch_bicg_jacobi_f32(st, d, S, G, flag, ln[c].ks, 1, r, v, sv, ln[c].db);
+                launch_apply_f32in(st, d, S, G, flag, ln[c].db, nullptr, t, 2, sv, ln[c].partial);
+            } else {
+                launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, 1, r, v, sv, sh);
+                launch_apply<double>(st, d, S, G, flag, sh, t, 2, sv, ln[c].partial);
+            }
             launch_finalize(st, 1, ln[c].partial, na, 2, ln[c].ks);
-            launch_bicg_xr<double>(st, d, ln[c].ks, ph, sh, sv, t, rh, s->us[c].p, r, ns, ln[c].partial);
+            if (f32)
+                launch_bicg_xr_f32dir(st, d, ln[c].ks, ln[c].da, ln[c].db, sv, t, rh, s->us[c].p, r, ns, ln[c].partial);
+            else
+                launch_bicg_xr<double>(st, d, ln[c].ks, ph, sh, sv, t, rh, s->us[c].p, r, ns, ln[c].partial);
             publish(c, 2, nv, 3);
         }
         for (int c = 0; c < 3; c++) {
