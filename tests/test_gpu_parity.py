@@ -276,7 +276,7 @@ def test_export_and_restart_roundtrip(new_gpu, cases, tmp_path):
         assert np.array_equal(nc.variables["flag"][:].transpose(2, 1, 0), g.flags())
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_sphere_runs_are_bitwise_reproducible(new_gpu, cases, precision):
     """Two solvers on the same stretched sphere case give identical bits.
 
@@ -292,6 +292,26 @@ def test_sphere_runs_are_bitwise_reproducible(new_gpu, cases, precision):
     assert runs[0][0] == runs[1][0]
     assert np.array_equal(runs[0][1], runs[1][1], equal_nan=True)
     assert np.array_equal(runs[0][2], runs[1][2], equal_nan=True)
+
+
+@pytest.mark.parametrize("precision", [0, 2], ids=["fp64", "fp64-krylov-fp32-mg"])
+def test_concurrent_momentum_solves_match_one_at_a_time(new_gpu, cases, precision):
+    """The three momentum solves run at once on three streams (the default)
+    or one after the other (under the per-kernel timers, which also launch
+    the multigrid cycles one kernel at a time instead of replaying graphs).
+    Each component's kernels and decisions are the same, so are the bits."""
+    runs = []
+    for timed in (False, True):
+        g = new_gpu()
+        cases.setup_sphere(g, 96, 48, 48, tol=1e-10, precision=precision)
+        if timed:
+            g.enable_timing(1)
+        g.iterate(2)
+        runs.append((g.last_iters().tolist(), g.last_norms().tolist(), g.get("U1"), g.get("U3"), g.get("P")))
+    assert runs[0][0] == runs[1][0]
+    assert runs[0][1] == runs[1][1]
+    for a, b in zip(runs[0][2:], runs[1][2:]):
+        assert np.array_equal(a, b, equal_nan=True)
 
 
 @pytest.mark.parametrize("smoother", [0, 1], ids=["fused-sweep", "per-colour"])
