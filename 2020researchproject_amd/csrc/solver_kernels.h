@@ -51,7 +51,9 @@ void launch_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const Gho
 
 // BiCGSTAB direction update fused with the Jacobi preconditioner:
 // which 0: p = r + beta (p - omega v), ph = D^-1 p; which 1: p = r - alpha v
-// (the s vector), ph = D^-1 p. Results equal launch_bicg_p / _s + launch_jacobi.
+// (the s vector), ph = D^-1 p; which 2: p = r (the first direction, equal to
+// which 0 with beta = 0), ph = D^-1 p. Results equal launch_bicg_p / _s +
+// launch_jacobi.
 template <typename T>
 void launch_bicg_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, const GhostRows<T> &G,
                         const uint8_t *flag, const KScal *ks, int which, const T *r, const T *v, T *p, T *ph);
@@ -64,7 +66,7 @@ void launch_bicg_jacobi_f32(hipStream_t st, const Dims &d, const SysCoef<double>
                             double *p, float *ph);
 
 // BiCGSTAB direction update in FP64 (which 0: p = r + beta (p - omega v);
-// 1: p = r - alpha v, the s vector) that also writes b32 = (float) p and,
+// 1: p = r - alpha v, the s vector; 2: p = r) that also writes b32 = (float) p and,
 // when x32 is not null, the first smoother colour on b32 with system *S
 // (the values launch_d2f_first writes).
 void launch_bicg_d2f(hipStream_t st, const Dims &d, const SysCoef<float> *S, const uint8_t *flag, const KScal *ks,
@@ -103,8 +105,10 @@ int launch_rbgs_color_ghost(hipStream_t st, const Dims &d, const SysCoef<T> &S, 
 
 // BiCGSTAB pieces
 template <typename T>
-// x = 0, r = rh = b, p = v = 0; partial (vec_partials blocks x 2): the
-// norms of r = b, ||r||^2 and ||N r||^2, as launch_residual gives for x = 0
+// x = 0, r = rh = b, p = v = 0 (p and v left alone when p is null: the
+// first direction is then formed with which = 2 below); partial
+// (vec_partials blocks x 2): the norms of r = b, ||r||^2 and ||N r||^2, as
+// launch_residual gives for x = 0
 void launch_bicg_init(hipStream_t st, const Dims &d, T *r, T *rh, T *p, T *v, T *x, const T *b, const NormSpec &ns,
                       double *partial);
 template <typename T>

@@ -478,6 +478,7 @@ __global__ void __launch_bounds__(256) k_jacobi(Dims d, SysCoef<T> S, GhostRows<
 // BiCGSTAB direction updates fused with the Jacobi preconditioner:
 //   SJ = 0: p = r + beta (p - omega v), ph = D^-1 p
 //   SJ = 1: s = r - alpha v,            sh = D^-1 s
+//   SJ = 2: p = r (the first direction), ph = D^-1 p
 // Same arithmetic as k_bicg_p / k_bicg_s followed by k_jacobi; ghost cells
 // (both outputs) are done by the ghost blocks. TPH: storage type of ph.
 template <typename T, int SJ, typename TPH = T>
@@ -492,7 +493,8 @@ __global__ void __launch_bounds__(256) k_bicg_jacobi(Dims d, SysCoef<T> S, Ghost
             const long c = G.cell[s];
             T n;
             if (SJ == 0) n = r[c] + (T)ks->beta * (p[c] - (T)ks->omega * v[c]);
-            else n = r[c] - (T)ks->alpha * v[c];
+            else if (SJ == 1) n = r[c] - (T)ks->alpha * v[c];
+            else n = r[c];
             p[c] = n;
             const T dg = ghost_diag(G, S.is_p, s);
             ph[c] = (TPH)(dg != T(0) ? n / dg : n);
@@ -505,14 +507,15 @@ __global__ void __launch_bounds__(256) k_bicg_jacobi(Dims d, SysCoef<T> S, Ghost
     const long q0 = cidx(d, i0, j, k);
     const unsigned fv = *reinterpret_cast<const unsigned short *>(flag + q0);
     const T2 rv = *reinterpret_cast<const T2 *>(r + q0);
-    const T2 vv = *reinterpret_cast<const T2 *>(v + q0);
-    T n0, n1;
+    T n0 = rv.x, n1 = rv.y;
     if (SJ == 0) {
+        const T2 vv = *reinterpret_cast<const T2 *>(v + q0);
         const T2 pv = *reinterpret_cast<const T2 *>(p + q0);
         const T beta = (T)ks->beta, omega = (T)ks->omega;
         n0 = rv.x + beta * (pv.x - omega * vv.x);
         n1 = rv.y + beta * (pv.y - omega * vv.y);
-    } else {
+    } else if (SJ == 1) {
+        const T2 vv = *reinterpret_cast<const T2 *>(v + q0);
         const T alpha = (T)ks->alpha;
         n0 = rv.x - alpha * vv.x;
         n1 = rv.y - alpha * vv.y;
@@ -548,18 +551,19 @@ __global__ void __launch_bounds__(256) k_bicg_d2f(Dims d, SysCoef<float> S, cons
     if (i0 >= d.nx || j >= d.ny) return;
     const long q0 = cidx(d, i0, j, k);
     const double2 rv = *reinterpret_cast<const double2 *>(r + q0);
-    const double2 vv = *reinterpret_cast<const double2 *>(v + q0);
     const unsigned fv = FIRST ? *reinterpret_cast<const unsigned short *>(flag + q0) : 0u;
     // diagonals of the pair (loaded ahead of the update)
     const float dg0 = FIRST ? stencil_diag(S, i0, j, k, q0) : 0.f;
     const float dg1 = FIRST ? stencil_diag(S, min(i0 + 1, d.nx - 1), j, k, q0 + 1) : 0.f;
-    double n0, n1;
+    double n0 = rv.x, n1 = rv.y;
     if (SJ == 0) {
+        const double2 vv = *reinterpret_cast<const double2 *>(v + q0);
         const double2 pv = *reinterpret_cast<const double2 *>(p + q0);
         const double beta = ks->beta, omega = ks->omega;
         n0 = rv.x + beta * (pv.x - omega * vv.x);
         n1 = rv.y + beta * (pv.y - omega * vv.y);
-    } else {
+    } else if (SJ == 1) {
+        const double2 vv = *reinterpret_cast<const double2 *>(v + q0);
         const double alpha = ks->alpha;
         n0 = rv.x - alpha * vv.x;
         n1 = rv.y - alpha * vv.y;
@@ -585,6 +589,9 @@ void launch_bicg_d2f(hipStream_t st, const Dims &d, const SysCoef<float> *S, con
     if (which == 0) {
         if (x32) hipLaunchKernelGGL((k_bicg_d2f<0, true>), jgrid(d), dim3(64, 4), 0, st, d, S0, flag, ks, r, v, p, b32, x32);
         else hipLaunchKernelGGL((k_bicg_d2f<0, false>), jgrid(d), dim3(64, 4), 0, st, d, S0, flag, ks, r, v, p, b32, x32);
+    } else if (which == 2) {
+        if (x32) hipLaunchKernelGGL((k_bicg_d2f<2, true>), jgrid(d), dim3(64, 4), 0, st, d, S0, flag, ks, r, v, p, b32, x32);
+        else hipLaunchKernelGGL((k_bicg_d2f<2, false>), jgrid(d), dim3(64, 4), 0, st, d, S0, flag, ks, r, v, p, b32, x32);
     } else {
         if (x32) hipLaunchKernelGGL((k_bicg_d2f<1, true>), jgrid(d), dim3(64, 4), 0, st, d, S0, flag, ks, r, v, p, b32, x32);
         else hipLaunchKernelGGL((k_bicg_d2f<1, false>), jgrid(d), dim3(64, 4), 0, st, d, S0, flag, ks, r, v, p, b32, x32);
@@ -598,8 +605,11 @@ void launch_bicg_jacobi(hipStream_t st, const Dims &d, const SysCoef<T> &S, cons
     if (which == 0)
         hipLaunchKernelGGL((k_bicg_jacobi<T, 0>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, ks, r, v, p, ph, ngb,
                            pgx(d), pgy(d));
-    else
+    else if (which == 1)
         hipLaunchKernelGGL((k_bicg_jacobi<T, 1>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, ks, r, v, p, ph, ngb,
+                           pgx(d), pgy(d));
+    else
+        hipLaunchKernelGGL((k_bicg_jacobi<T, 2>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, ks, r, v, p, ph, ngb,
                            pgx(d), pgy(d));
 }
 
@@ -610,8 +620,11 @@ void launch_bicg_jacobi_f32(hipStream_t st, const Dims &d, const SysCoef<double>
     if (which == 0)
         hipLaunchKernelGGL((k_bicg_jacobi<double, 0, float>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, ks, r, v, p,
                            ph, ngb, pgx(d), pgy(d));
-    else
+    else if (which == 1)
         hipLaunchKernelGGL((k_bicg_jacobi<double, 1, float>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, ks, r, v, p,
+                           ph, ngb, pgx(d), pgy(d));
+    else
+        hipLaunchKernelGGL((k_bicg_jacobi<double, 2, float>), dim3(nb), dim3(64, 4), 0, st, d, S, G, flag, ks, r, v, p,
                            ph, ngb, pgx(d), pgy(d));
 }
 
@@ -989,7 +1002,8 @@ __global__ void __launch_bounds__(256) k_bicg_init(Dims d, T *r, T *rh, T *p, T 
         for (int k = k0; k < k1; k++) {
             const long q = cidx(d, i, j, k);
             const T bv = b[q];
-            r[q] = bv; rh[q] = bv; p[q] = 0; v[q] = 0; x[q] = 0;
+            r[q] = bv; rh[q] = bv; x[q] = 0;
+            if (p) { p[q] = 0; v[q] = 0; }
             const double bd = (double)bv, bs = bd * nscale(ns, d, i, j, k, q);
             acc[0] += bd * bd;
             acc[1] += bs * bs;

@@ -751,9 +751,14 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
     const double cb = ncells(s) * sizeof(T);
     char nm[64];
 
+    // the fused direction updates (Jacobi, FP32 multigrid) form the first
+    // direction as p = r, so p and v need no clearing
+    const bool fused_dir = prec == 1 || (prec == 3 && std::is_same<T, double>::value);
     // x0 = 0: r0 = b, whose norms the initialisation kernel forms
-    TIMED("bicg_init", 6 * cb, launch_bicg_init<T>(s->stream, d, r, rh, p, v, x, b, ns, s->partial.p));
+    TIMED("bicg_init", (fused_dir ? 4 : 6) * cb,
+          launch_bicg_init<T>(s->stream, d, r, rh, fused_dir ? nullptr : p, v, x, b, ns, s->partial.p));
     finalize_read(s, 3, nv, 2);
+    bool first_dir = fused_dir;
     const double bn = bnorm > 0 ? bnorm : std::sqrt(ns.on ? s->ks_h->bbn : s->ks_h->bb);
     if (bn == 0) { *final_rel = 0; return 0; }
     double rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
@@ -784,6 +789,9 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
     for (;;) {
     while (it < maxit && rn > tol * bn) {
         it++;
+        // which direction update: 2 forms the first direction p = r
+        const int wp = first_dir ? 2 : 0;
+        first_dir = false;
         snprintf(nm, sizeof nm, "spmv_%s", tag);
         // direction p and ph = M^-1 p, then v = A ph. FP32 multigrid: the
         // update writes the FP32 right-hand side (and the cycle's first
@@ -791,16 +799,16 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         if (prec == 1) {
             if constexpr (std::is_same<T, double>::value) {
                 if (f32mom)
-                    TIMED("bicg_jacobi", 4.5 * cb, launch_bicg_jacobi_f32(s->stream, d, S, G, flag, s->ks.p, 0, r, v, p,
+                    TIMED("bicg_jacobi", 4.5 * cb, launch_bicg_jacobi_f32(s->stream, d, S, G, flag, s->ks.p, wp, r, v, p,
                                                                         e32a));
                 else
-                    TIMED("bicg_jacobi", 5 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 0, r, v, p, ph));
+                    TIMED("bicg_jacobi", 5 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, wp, r, v, p, ph));
             } else {
-                TIMED("bicg_jacobi", 5 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, 0, r, v, p, ph));
+                TIMED("bicg_jacobi", 5 * cb, launch_bicg_jacobi<T>(s->stream, d, S, G, flag, s->ks.p, wp, r, v, p, ph));
             }
         } else if (f32mg) {
             if constexpr (std::is_same<T, double>::value) {
-                TIMED("bicg_d2f", (fuse1 ? 5.5 : 5) * cb, launch_bicg_d2f(s->stream, d, S32f, flag, s->ks.p, 0, r, v, p,
+                TIMED("bicg_d2f", (fuse1 ? 5.5 : 5) * cb, launch_bicg_d2f(s->stream, d, S32f, flag, s->ks.p, wp, r, v, p,
                                                                      s->ref_rhs32.p, fuse1 ? e32a : nullptr));
                 mg_vcycle_f32(s, fuse1, e32a);
             }
@@ -1186,7 +1194,8 @@ static void momentum_concurrent(IBMSolver *s, const double bnorm[3]) {
     bool act[3];
     for (int c = 0; c < 3; c++) {
         DBuf<double> *V = ln[c].V;
-        launch_bicg_init<double>(ln[c].st, d, V[0].p, V[1].p, V[2].p, V[3].p, s->us[c].p, s->b[c].p, ns,
+        // p and v need no clearing: the first direction is formed as p = r
+        launch_bicg_init<double>(ln[c].st, d, V[0].p, V[1].p, nullptr, V[3].p, s->us[c].p, s->b[c].p, ns,
                                  ln[c].partial);
         publish(c, 3, nv, 2);
     }
@@ -1212,11 +1221,12 @@ static void momentum_concurrent(IBMSolver *s, const double bnorm[3]) {
             double *r = V[0].p, *rh = V[1].p, *p = V[2].p, *v = V[3].p, *ph = V[4].p, *sv = V[5].p, *sh = V[6].p,
                    *t = V[7].p;
             const hipStream_t st = ln[c].st;
+            const int wp = it[c] == 1 ? 2 : 0;  // the first direction: p = r
             if (f32) {
-                launch_bicg_jacobi_f32(st, d, S, G, flag, ln[c].ks, 0, r, v, p, ln[c].da);
+                launch_bicg_jacobi_f32(st, d, S, G, flag, ln[c].ks, wp, r, v, p, ln[c].da);
                 launch_apply_f32in(st, d, S, G, flag, ln[c].da, nullptr, v, 1, rh, ln[c].partial);
             } else {
-                launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, 0, r, v, p, ph);
+                launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, wp, r, v, p, ph);
                 launch_apply<double>(st, d, S, G, flag, ph, v, 1, rh, ln[c].partial);
             }
             launch_finalize(st, 0, ln[c].partial, na, 1, ln[c].ks);
