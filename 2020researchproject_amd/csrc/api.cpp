@@ -56,7 +56,12 @@ IBMSolver *IBMSolver_new(const int num_process, const int rank, const int Px, co
     HIP_CHECK(hipHostMalloc((void **)&s->pub_h, sizeof(KPub), hipHostMallocCoherent | hipHostMallocMapped));
     memset((void *)s->pub_h, 0, sizeof(KPub));
     HIP_CHECK(hipHostGetDevicePointer((void **)&s->pub_d, s->pub_h, 0));
-    HIP_CHECK(hipHostMalloc((void **)&s->sums_h, kSumSlots * sizeof(double), hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc((void **)&s->sums_h, (kSumSlots + 1) * sizeof(double),
+                            hipHostMallocCoherent | hipHostMallocMapped));
+    memset((void *)s->sums_h, 0, (kSumSlots + 1) * sizeof(double));
+    HIP_CHECK(hipHostGetDevicePointer((void **)&s->sums_d, s->sums_h, 0));
+    s->sums_h_epoch = reinterpret_cast<unsigned *>(s->sums_h + kSumSlots);
+    s->sums_d_epoch = reinterpret_cast<unsigned *>(s->sums_d + kSumSlots);
     HIP_CHECK(hipHostMalloc((void **)&s->gm_h, 64 * sizeof(double), hipHostMallocDefault));
     s->ks.alloc(1);
     s->sums.alloc(kSumSlots);

@@ -244,6 +244,11 @@ struct _ibm_solver {
     ibm::DBuf<double> conv_g[6];     // BC_CONVECTIVE_OUTFLOW face values (4 comps, bc[] order)
     bool conv_ready = false;         // conv_g hold the current face values
     ibm::DBuf<double> partial, sums;  // sums: kSumSlots doubles
+    // sums_h: device-mapped coherent host memory, kSumSlots doubles then an
+    // epoch; sums_d / sums_d_epoch: the same memory's device addresses
+    double *sums_d = nullptr;
+    unsigned *sums_h_epoch = nullptr, *sums_d_epoch = nullptr;
+    unsigned sums_epoch = 0;
     ibm::DBuf<ibm::KScal> ks;
     ibm::KScal *ks_h = nullptr;      // pinned
     ibm::KPub *pub_h = nullptr;      // host address (see KPub)

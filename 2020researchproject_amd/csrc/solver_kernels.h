@@ -164,6 +164,9 @@ void launch_lincomb(hipStream_t st, const Dims &d, const T *z, double f, T *y, i
 // null): the scalars are also copied there, then *pub_epoch = epoch.
 void launch_finalize(hipStream_t st, int op, const double *partial, long nblk, int nv, KScal *ks,
                      KScal *pub = nullptr, unsigned *pub_epoch = nullptr, unsigned epoch = 0);
+// out[0 .. nv) = in[0 .. nv) into device-mapped host memory, then *epoch_out
+// = epoch (after a system-scope fence), for a host thread waiting on it
+void launch_publish(hipStream_t st, const double *in, int nv, double *out, unsigned *epoch_out, unsigned epoch);
 // out[a] = sum (or max) over blocks of partial[b * nv + a]
 void launch_sum_partials(hipStream_t st, const double *partial, long nblk, int nv, double *out, int reduce_max);
 

@@ -1180,6 +1180,18 @@ __global__ void __launch_bounds__(kFinThreads) k_finalize(int op, const double *
 }
 
 // sums nv partial columns into out[0..nv) (same fixed order as k_finalize)
+// out[0 .. nv) = in[0 .. nv), then *epoch_out = epoch (system-scope fence
+// between): a publication into device-mapped host memory
+__global__ void k_publish(const double *in, int nv, double *out, unsigned *epoch_out, unsigned epoch) {
+    if (threadIdx.x != 0) return;
+    for (int a = 0; a < nv; a++) out[a] = in[a];
+    __threadfence_system();
+    *reinterpret_cast<volatile unsigned *>(epoch_out) = epoch;
+}
+void launch_publish(hipStream_t st, const double *in, int nv, double *out, unsigned *epoch_out, unsigned epoch) {
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, in, nv, out, epoch_out, epoch);
+}
+
 __global__ void __launch_bounds__(kFinThreads) k_sum_partials(const double *partial, long nblk, int nv, double *out,
                                                               int reduce_max) {
     __shared__ double sh[4][kFinThreads];

@@ -184,16 +184,10 @@ static void update_ghost(IBMSolver *s) {
                               s->ghost_old.p, s->u[0].p, s->u[1].p, s->u[2].p, s->p.p));
 }
 
-static void read_sums(IBMSolver *s, int nv) {
-    HIP_CHECK(hipMemcpyAsync(s->sums_h, s->sums.p, nv * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    HIP_CHECK(hipStreamSynchronize(s->stream));
-}
-
-// Waits until the finalize kernel enqueued on `st` has published epoch e in
-// pub_h, polling the stream now and then so that an error or a lost
+// Waits until a kernel enqueued on `st` has published epoch e at *ep (host
+// memory), polling the stream now and then so that an error or a lost
 // publication cannot make the wait endless.
-static void wait_published(hipStream_t st, KPub *pub_h, unsigned e) {
-    unsigned *ep = &pub_h->epoch;
+static void wait_epoch(hipStream_t st, unsigned *ep, unsigned e) {
     for (long n = 1; __atomic_load_n(ep, __ATOMIC_ACQUIRE) != e; n++) {
         if ((n & 1023) == 0) {
             const hipError_t q = hipStreamQuery(st);
@@ -207,6 +201,15 @@ static void wait_published(hipStream_t st, KPub *pub_h, unsigned e) {
         }
         __builtin_ia32_pause();
     }
+}
+static void wait_published(hipStream_t st, KPub *pub_h, unsigned e) { wait_epoch(st, &pub_h->epoch, e); }
+
+// The first nv sums, copied by a small kernel into device-mapped host memory
+// (sums_h) and awaited there
+static void read_sums(IBMSolver *s, int nv) {
+    const unsigned e = ++s->sums_epoch;
+    launch_publish(s->stream, s->sums.p, nv, s->sums_d, s->sums_d_epoch, e);
+    wait_epoch(s->stream, s->sums_h_epoch, e);
 }
 
 // Finalize kernel `op` over the block partials, with the resulting scalars
@@ -1393,14 +1396,12 @@ static void do_step(IBMSolver *s) {
     if (s->mom_lanes && s->timing == 0 && s->precision != IBM3D_PREC_MIXED) {
         const double bn[3] = {std::sqrt(bn2[0]), std::sqrt(bn2[1]), std::sqrt(bn2[2])};
         momentum_concurrent(s, bn);
-        for (int l = 0; l < 3; l++)
-            TIMED("outer_sol", 0, launch_outer(s->stream, s->geo, ob, s->us[l].p, nullptr, nullptr, 2, l));
     } else {
-        for (int l = 0; l < 3; l++) {
-            solve_system(s, l, s->b[l].p, s->us[l].p, std::sqrt(bn2[l]));
-            TIMED("outer_sol", 0, launch_outer(s->stream, s->geo, ob, s->us[l].p, nullptr, nullptr, 2, l));
-        }
+        for (int l = 0; l < 3; l++) solve_system(s, l, s->b[l].p, s->us[l].p, std::sqrt(bn2[l]));
     }
+    // outer layers of the three solutions from their BC rows (the per-component
+    // fill of each u* with the solve's boundary data, all three in one pass)
+    TIMED("outer_sol", 0, launch_outer(s->stream, s->geo, ob, s->us[0].p, s->us[1].p, s->us[2].p, 0, 0));
     // K2: U*, Poisson RHS
     {
         const double *us3[3] = {s->us[0].p, s->us[1].p, s->us[2].p};
