@@ -286,7 +286,8 @@ struct _ibm_solver {
     int final_iter[4] = {0, 0, 0, 0};
     int timing = 0;             // 0 off, 1 every kernel (eager), 2 kernels + whole V-cycle graphs
     bool graphs = true;         // replay multigrid cycles as HIP graphs
-    std::vector<double> step_ms;          // wall time of each step of the last iterate call
+    std::vector<double> step_ms;          // each step of the last iterate call: device time between
+                                          // the ends of consecutive steps (ms)
     std::vector<ibm::KTimer> timers;
     std::vector<hipEvent_t> event_pool;
 };

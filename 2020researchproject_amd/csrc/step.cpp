@@ -1465,8 +1465,12 @@ void iterate_impl(IBMSolver *s, int steps, bool verbose) {
         launch_conv_rhs(s->stream, s->geo, sp, s->flag.p, u3, U3, s->p.p, Np, N, b, tp, s->partial.p);
         for (int l = 0; l < 3; l++) std::swap(s->N[l], s->Np[l]);
     }
+    // step times: events recorded on the stream at each step's end (the host
+    // runs ahead into the next step instead of draining the stream)
     s->step_ms.clear();
-    auto tprev = std::chrono::steady_clock::now();
+    std::vector<hipEvent_t> ends;
+    hipEvent_t start = get_event(s);
+    HIP_CHECK(hipEventRecord(start, s->stream));
     for (int i = 0; i < steps;) {
         do_step(s);
         i++;
@@ -1474,10 +1478,8 @@ void iterate_impl(IBMSolver *s, int steps, bool verbose) {
         s->time = t0 + s->dt * i;
         update_outer(s);
         update_ghost(s);
-        HIP_CHECK(hipStreamSynchronize(s->stream));
-        const auto tnow = std::chrono::steady_clock::now();
-        s->step_ms.push_back(std::chrono::duration<double, std::milli>(tnow - tprev).count());
-        tprev = tnow;
+        ends.push_back(get_event(s));
+        HIP_CHECK(hipEventRecord(ends.back(), s->stream));
         if (verbose) {
             if (s->iter % 10 == 1 || i == 1) {
                 printf("\n  iter       u1 res       u2 res       u3 res        p res   its(u1 u2 u3 p)\n");
@@ -1495,6 +1497,15 @@ void iterate_impl(IBMSolver *s, int steps, bool verbose) {
         }
     }
     HIP_CHECK(hipStreamSynchronize(s->stream));
+    hipEvent_t prev = start;
+    for (hipEvent_t e : ends) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, prev, e));
+        s->step_ms.push_back(ms);
+        s->event_pool.push_back(prev);
+        prev = e;
+    }
+    s->event_pool.push_back(prev);
 }
 
 double max_div_impl(IBMSolver *s) {

@@ -96,9 +96,11 @@ double IBMSolver_max_div(IBMSolver *);
    (Fx, Fy, Fz); drag and lift coefficients follow by scaling. */
 void IBMSolver_calc_body_force(IBMSolver *, double out[3]);
 
-/* Wall-clock milliseconds of each time step of the last IBMSolver_iterate
-   call (host clock, the step's device work complete at its end): writes up
-   to max values, returns the number of steps. */
+/* Milliseconds of each time step of the last IBMSolver_iterate call: the
+   time between the completions of consecutive steps on the device (HIP
+   events recorded on the solver's stream at each step's end; the first step
+   is measured from the start of the call's device work). Writes up to max
+   values, returns the number of steps. */
 int IBMSolver_get_step_times(IBMSolver *, double *out, int max);
 
 /* Timing with HIP events on the solver's stream. mode 0: off; 1: every
