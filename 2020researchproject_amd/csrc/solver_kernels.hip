@@ -1106,29 +1106,39 @@ __device__ inline void sum_columns(const double *partial, long nblk, int nv, int
                                    double (&sh)[4][kFinThreads]) {
     const int t = threadIdx.x;
     double acc[4] = {0, 0, 0, 0};
-    long b = t;
-    // four rows in flight per thread
-    for (; b + 3 * kFinThreads < nblk; b += 4 * kFinThreads) {
+    // four rows per thread in flight at once, the rows past nblk predicated
+    // off (a grid of up to 4 kFinThreads blocks takes one round of loads)
+    for (long b = t; b < nblk; b += 4 * kFinThreads) {
         double v[4][4];
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-            for (int a = 0; a < nv; a++) v[u][a] = partial[(b + u * kFinThreads) * nv + a];
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-            for (int a = 0; a < nv; a++) acc[a] = reduce_max ? fmax(acc[a], v[u][a]) : acc[a] + v[u][a];
-    }
-    for (; b < nblk; b += kFinThreads)
-        for (int a = 0; a < nv; a++) {
-            const double v = partial[b * nv + a];
-            acc[a] = reduce_max ? fmax(acc[a], v) : acc[a] + v;
+        for (int u = 0; u < 4; u++) {
+            const long bu = b + u * kFinThreads;
+            for (int a = 0; a < nv; a++) v[u][a] = bu < nblk ? partial[bu * nv + a] : 0.0;
         }
-    for (int a = 0; a < nv; a++) sh[a][t] = acc[a];
-    __syncthreads();
-    for (int s = kFinThreads / 2; s > 0; s >>= 1) {
-        if (t < s)
-            for (int a = 0; a < nv; a++) sh[a][t] = reduce_max ? fmax(sh[a][t], sh[a][t + s]) : sh[a][t] + sh[a][t + s];
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (b + u * kFinThreads >= nblk) break;
+            for (int a = 0; a < nv; a++) acc[a] = reduce_max ? fmax(acc[a], v[u][a]) : acc[a] + v[u][a];
+        }
     }
+    // fixed-shape tree: within each wave by shuffles, then over the waves
+    const int lane = t & 63, wave = t >> 6;
+    for (int a = 0; a < nv; a++) {
+        double x = acc[a];
+        for (int o = 32; o > 0; o >>= 1) {
+            const double y = __shfl_down(x, o, 64);
+            x = reduce_max ? fmax(x, y) : x + y;
+        }
+        if (lane == 0) sh[a][wave] = x;
+    }
+    __syncthreads();
+    if (t == 0)
+        for (int a = 0; a < nv; a++) {
+            double x = sh[a][0];
+            for (int w = 1; w < kFinThreads / 64; w++) x = reduce_max ? fmax(x, sh[a][w]) : x + sh[a][w];
+            sh[a][0] = x;
+        }
+    __syncthreads();
 }
 
 __global__ void __launch_bounds__(kFinThreads) k_finalize(int op, const double *partial, long nblk, int nv,
