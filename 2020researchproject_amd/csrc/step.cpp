@@ -1148,9 +1148,9 @@ static void report_solve(IBMSolver *s, int sk, double rel, int it) {
 // The three momentum solves at once (IBM3D_PREC_FP64 and _MIXED_KRYLOV):
 // BiCGSTAB with Jacobi preconditioning from x0 = 0, component 0 on the
 // solver's stream and buffers, components 1 and 2 on lanes of their own.
-// Each iteration is enqueued for every active component before the host
-// waits for any of their scalars, so the components' kernels, reductions and
-// host reads overlap on the device. Per component: the kernels and decisions
+// The host polls the lanes' published scalars and enqueues a component's next
+// iteration as soon as its own are in, so the components' kernels, reductions
+// and host reads overlap on the device and no lane waits for another. Per component: the kernels and decisions
 // of bicgstab<double> with prec 1 and no restarts.
 // ---------------------------------------------------------------------------
 static void momentum_concurrent(IBMSolver *s, const double bnorm[3]) {
@@ -1209,50 +1209,76 @@ static void momentum_concurrent(IBMSolver *s, const double bnorm[3]) {
         act[c] = bn[c] != 0;
     }
     const bool zero[3] = {!act[0], !act[1], !act[2]};
-    for (;;) {
+    // one BiCGSTAB iteration of component c, enqueued on its lane
+    auto enqueue_iteration = [&](int c) {
+        it[c]++;
+        const SysCoef<double> &S = s->sys[c].S64;
+        DBuf<double> *V = ln[c].V;
+        double *r = V[0].p, *rh = V[1].p, *p = V[2].p, *v = V[3].p, *ph = V[4].p, *sv = V[5].p, *sh = V[6].p,
+               *t = V[7].p;
+        const hipStream_t st = ln[c].st;
+        const int wp = it[c] == 1 ? 2 : 0;  // the first direction: p = r
+        if (f32) {
+            launch_bicg_jacobi_f32(st, d, S, G, flag, ln[c].ks, wp, r, v, p, ln[c].da);
+            launch_apply_f32in(st, d, S, G, flag, ln[c].da, nullptr, v, 1, rh, ln[c].partial);
+        } else {
+            launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, wp, r, v, p, ph);
+            launch_apply<double>(st, d, S, G, flag, ph, v, 1, rh, ln[c].partial);
+        }
+        launch_finalize(st, 0, ln[c].partial, na, 1, ln[c].ks);
+        if (f32) {
+            launch_bicg_jacobi_f32(st, d, S, G, flag, ln[c].ks, 1, r, v, sv, ln[c].db);
+            launch_apply_f32in(st, d, S, G, flag, ln[c].db, nullptr, t, 2, sv, ln[c].partial);
+        } else {
+            launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, 1, r, v, sv, sh);
+            launch_apply<double>(st, d, S, G, flag, sh, t, 2, sv, ln[c].partial);
+        }
+        launch_finalize(st, 1, ln[c].partial, na, 2, ln[c].ks);
+        if (f32)
+            launch_bicg_xr_f32dir(st, d, ln[c].ks, ln[c].da, ln[c].db, sv, t, rh, s->us[c].p, r, ns, ln[c].partial);
+        else
+            launch_bicg_xr<double>(st, d, ln[c].ks, ph, sh, sv, t, rh, s->us[c].p, r, ns, ln[c].partial);
+        publish(c, 2, nv, 3);
+    };
+    // a component goes on while it is not converged, not broken down and
+    // under maxit
+    auto goes_on = [&](int c) { return act[c] && it[c] < maxit && rn[c] > s->tol * bn[c]; };
+    // Each lane's next iteration is enqueued as soon as that lane's scalars
+    // are in, so the lanes do not wait for one another; a component's own
+    // kernels and decisions stay those of the one-at-a-time loop.
+    bool fly[3] = {false, false, false};
+    for (int c = 0; c < 3; c++) {
+        act[c] = goes_on(c);
+        if (act[c]) { enqueue_iteration(c); fly[c] = true; }
+    }
+    for (long n = 1;; n++) {
         bool any = false;
         for (int c = 0; c < 3; c++) {
-            act[c] = act[c] && it[c] < maxit && rn[c] > s->tol * bn[c];
-            any = any || act[c];
-        }
-        if (!any) break;
-        for (int c = 0; c < 3; c++) {
-            if (!act[c]) continue;
-            it[c]++;
-            const SysCoef<double> &S = s->sys[c].S64;
-            DBuf<double> *V = ln[c].V;
-            double *r = V[0].p, *rh = V[1].p, *p = V[2].p, *v = V[3].p, *ph = V[4].p, *sv = V[5].p, *sh = V[6].p,
-                   *t = V[7].p;
-            const hipStream_t st = ln[c].st;
-            const int wp = it[c] == 1 ? 2 : 0;  // the first direction: p = r
-            if (f32) {
-                launch_bicg_jacobi_f32(st, d, S, G, flag, ln[c].ks, wp, r, v, p, ln[c].da);
-                launch_apply_f32in(st, d, S, G, flag, ln[c].da, nullptr, v, 1, rh, ln[c].partial);
-            } else {
-                launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, wp, r, v, p, ph);
-                launch_apply<double>(st, d, S, G, flag, ph, v, 1, rh, ln[c].partial);
+            if (!fly[c]) continue;
+            any = true;
+            if (__atomic_load_n(&ln[c].pub_h->epoch, __ATOMIC_ACQUIRE) != ep[c]) {
+                // now and then: an idle lane stream with nothing published
+                // (or an error) ends the wait instead of making it endless
+                if ((n & 1023) == 0) {
+                    const hipError_t q = hipStreamQuery(ln[c].st);
+                    if (q == hipSuccess && __atomic_load_n(&ln[c].pub_h->epoch, __ATOMIC_ACQUIRE) != ep[c]) {
+                        fprintf(stderr, "error: Krylov scalars were not published (epoch %u, expected %u)\n",
+                                __atomic_load_n(&ln[c].pub_h->epoch, __ATOMIC_ACQUIRE), ep[c]);
+                        abort();
+                    }
+                    if (q != hipSuccess && q != hipErrorNotReady) HIP_CHECK(q);
+                }
+                continue;
             }
-            launch_finalize(st, 0, ln[c].partial, na, 1, ln[c].ks);
-            if (f32) {
-                launch_bicg_jacobi_f32(st, d, S, G, flag, ln[c].ks, 1, r, v, sv, ln[c].db);
-                launch_apply_f32in(st, d, S, G, flag, ln[c].db, nullptr, t, 2, sv, ln[c].partial);
-            } else {
-                launch_bicg_jacobi<double>(st, d, S, G, flag, ln[c].ks, 1, r, v, sv, sh);
-                launch_apply<double>(st, d, S, G, flag, sh, t, 2, sv, ln[c].partial);
-            }
-            launch_finalize(st, 1, ln[c].partial, na, 2, ln[c].ks);
-            if (f32)
-                launch_bicg_xr_f32dir(st, d, ln[c].ks, ln[c].da, ln[c].db, sv, t, rh, s->us[c].p, r, ns, ln[c].partial);
-            else
-                launch_bicg_xr<double>(st, d, ln[c].ks, ph, sh, sv, t, rh, s->us[c].p, r, ns, ln[c].partial);
-            publish(c, 2, nv, 3);
-        }
-        for (int c = 0; c < 3; c++) {
-            if (!act[c]) continue;
             collect(c);
+            fly[c] = false;
             rn[c] = std::sqrt(kh[c].rr);
             if (kh[c].breakdown || !std::isfinite(rn[c])) act[c] = false;
+            act[c] = goes_on(c);
+            if (act[c]) { enqueue_iteration(c); fly[c] = true; }
         }
+        if (!any) break;
+        __builtin_ia32_pause();
     }
     // true residuals of the final iterates
     for (int c = 0; c < 3; c++) {
