@@ -1153,7 +1153,11 @@ static void report_solve(IBMSolver *s, int sk, double rel, int it) {
 // and host reads overlap on the device and no lane waits for another. Per component: the kernels and decisions
 // of bicgstab<double> with prec 1 and no restarts.
 // ---------------------------------------------------------------------------
-static void momentum_concurrent(IBMSolver *s, const double bnorm[3]) {
+// norms(bnorm) gives the three stopping norms (a host read of the step's
+// sums), called once the initialisations are enqueued so that the read
+// overlaps them.
+template <typename F>
+static void momentum_concurrent(IBMSolver *s, F &&norms) {
     const Dims &d = s->d;
     const GhostRows<double> &G = s->gmom.G64;
     const uint8_t *flag = s->flag.p;
@@ -1202,6 +1206,8 @@ static void momentum_concurrent(IBMSolver *s, const double bnorm[3]) {
                                  ln[c].partial);
         publish(c, 3, nv, 2);
     }
+    double bnorm[3];
+    norms(bnorm);
     for (int c = 0; c < 3; c++) {
         collect(c);
         bn[c] = bnorm[c] > 0 ? bnorm[c] : std::sqrt(kh[c].bb);
@@ -1409,21 +1415,26 @@ static void do_step(IBMSolver *s) {
                                  : ax == 1 ? (hi ? s->g1d[9].p : s->g1d[8].p) : (hi ? s->g1d[11].p : s->g1d[10].p);
                 launch_bc_rhs(s->stream, s->geo, ob, ax, hi, l, Kf, gm, s->b[l].p);
             }
-    read_sums(s, conv ? kConvNormSlot + 18 : 3);
-    double bn2[3];
-    for (int l = 0; l < 3; l++) {
-        bn2[l] = s->sums_h[l] + s->sys[l].bc_norm2 + fnorm[l];
-        for (int f = 0; conv && f < 6; f++)
-            if (s->bc[f].type == BC_CONVECTIVE_OUTFLOW) bn2[l] += s->sums_h[kConvNormSlot + 3 * f + l];
-    }
+    // stopping norms of the momentum solves: the right-hand sides' sums
+    // (host read) plus the BC rows' and forcing parts
+    auto mom_norms = [&](double bn[3]) {
+        read_sums(s, conv ? kConvNormSlot + 18 : 3);
+        for (int l = 0; l < 3; l++) {
+            double bn2 = s->sums_h[l] + s->sys[l].bc_norm2 + fnorm[l];
+            for (int f = 0; conv && f < 6; f++)
+                if (s->bc[f].type == BC_CONVECTIVE_OUTFLOW) bn2 += s->sums_h[kConvNormSlot + 3 * f + l];
+            bn[l] = std::sqrt(bn2);
+        }
+    };
     // momentum Helmholtz solves: the three at once on their lanes, or one
     // after the other (mixed defect-correction mode, and under the per-kernel
     // timers, whose events are recorded on the solver's stream)
     if (s->mom_lanes && s->timing == 0 && s->precision != IBM3D_PREC_MIXED) {
-        const double bn[3] = {std::sqrt(bn2[0]), std::sqrt(bn2[1]), std::sqrt(bn2[2])};
-        momentum_concurrent(s, bn);
+        momentum_concurrent(s, mom_norms);
     } else {
-        for (int l = 0; l < 3; l++) solve_system(s, l, s->b[l].p, s->us[l].p, std::sqrt(bn2[l]));
+        double bn[3];
+        mom_norms(bn);
+        for (int l = 0; l < 3; l++) solve_system(s, l, s->b[l].p, s->us[l].p, bn[l]);
     }
     // outer layers of the three solutions from their BC rows (the per-component
     // fill of each u* with the solve's boundary data, all three in one pass)
