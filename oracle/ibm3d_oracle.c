@@ -407,6 +407,7 @@ static void conv_outflow(IBMSolver *s, int init) {
 /* ------------------------------------------------------------------------ */
 
 static void calc_lvset_flag(IBMSolver *s);
+static void calc_flag(IBMSolver *s);
 static void build_matrices(IBMSolver *s);
 
 void IBMSolver_assemble(IBMSolver *s) {
@@ -497,7 +498,12 @@ static void calc_lvset_flag(IBMSolver *s) {
                     C3(s, s->lvset, i, j, k) = oracle_lvset_sphere(
                         s->sphere, C1(s->xc, i), C1(s->yc, j), C1(s->zc, k));
     }
-    /* flags, :1235-1254 */
+    calc_flag(s);
+}
+
+/* flags from the level set  <- src/ibm3d_setup.c:1235-1254 */
+static void calc_flag(IBMSolver *s) {
+    const int Nx = s->Nx, Ny = s->Ny, Nz = s->Nz;
     for (int i = -2; i < Nx + 2; i++)
         for (int j = -2; j < Ny + 2; j++)
             for (int k = -2; k < Nz + 2; k++) {
@@ -1632,6 +1638,14 @@ void oracle_get_kcoef(IBMSolver *s, double *out6) {
 void oracle_update_outer(IBMSolver *s) { update_outer(s); }
 /* Runs only the ghost-cell pass (update_ghost) on the current fields. */
 void oracle_update_ghost(IBMSolver *s) { update_ghost(s); }
+/* Replaces the level set (reference layout, P values) and recomputes the
+   flags from it by the rule of calc_lvset_flag; the assembled matrices are
+   left as they were. Test hook for the ghost-cell pass on bodies that have no
+   STL (e.g. a plane). */
+void oracle_set_lvset(IBMSolver *s, const double *phi) {
+    memcpy(s->lvset, phi, ncell(s) * sizeof(double));
+    calc_flag(s);
+}
 /* Runs calc_N on the current fields after interp_stag_vel; the result is
    returned through IBM3D_FIELD_N* (stored in the N_prev slot). */
 void oracle_calc_N(IBMSolver *s) {

@@ -436,3 +436,174 @@ def test_smagorinsky_closed_form_linear_shear(I, new_oracle, cases, model, cs):
     expect = -2 * l**2 * abs(a) * (a / 2)
     np.testing.assert_allclose(t12, np.broadcast_to(expect, t12.shape), rtol=1e-12, atol=0)
     assert np.allclose(s.get("TAU13")[2:-2, 2:-2, 2:-2], 0) and np.allclose(s.get("TAU11")[2:-2, 2:-2, 2:-2], 0)
+
+
+def test_vreman_closed_form_constant_gradient(I, new_oracle, cases):
+    """Vreman (extension; Vreman, Phys. Fluids 16, 3670, 2004) on a velocity
+    field with a constant, non-symmetric gradient alpha_mn = du_n/dx_m on a
+    grid stretched in y: central differences are exact for linear fields, so
+    per cell nu_t = c sqrt(B / |alpha|^2) with beta = alpha^T D^2 alpha (D =
+    diag of the cell widths) and B = ((tr beta)^2 - tr(beta^2)) / 2, the second
+    invariant of beta (the oracle sums the 2x2 principal minors instead);
+    tau = -2 nu_t S. A transposed alpha, a missing width or the cube-root
+    filter width of Smagorinsky all change tau here."""
+    c = 0.07
+    G = np.array([[0.3, -1.1, 0.4], [2.0, 0.2, -0.7], [0.5, 0.9, -0.5]])  # G[m, n] = du_n/dx_m
+    xf = cases.uniform_faces(0.0, 1.2, 6)
+    yf = np.array([-math.cos(math.pi * j / 8) for j in range(9)])
+    zf = cases.uniform_faces(0.0, 0.5, 4)
+    s = new_oracle()
+
+    def bc(s):
+        s.set_bc(I.DIR_EAST | I.DIR_WEST | I.DIR_NORTH | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP,
+                 I.BC_STATIONARY_WALL)
+        s.set_turb_model(I.TURBMODEL_VREMAN, c)
+
+    box(I, s, xf, yf, zf, Re=100.0, dt=0.01, bc=bc)
+    bind_oracle_hooks(s.lib)
+    xc, yc, zc, dx, dy, dz = coords(s)
+    X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+    s.init_flow_const(0, 0, 0, 0)
+    for n, name in enumerate(("U1", "U2", "U3")):
+        s.set(name, G[0, n] * X + G[1, n] * Y + G[2, n] * Z)
+    s.lib.IBMSolver_calc_tau_r(s.h)
+    S = 0.5 * (G + G.T)
+    got = {nm: s.get(nm)[2:-2, 2:-2, 2:-2] for nm in ("TAU11", "TAU12", "TAU13", "TAU22", "TAU23", "TAU33")}
+    for i in range(len(xf) - 1):
+        for j in range(len(yf) - 1):
+            for k in range(len(zf) - 1):
+                D2 = np.diag([dx[i + 2] ** 2, dy[j + 2] ** 2, dz[k + 2] ** 2])
+                beta = G.T @ D2 @ G
+                B = 0.5 * (np.trace(beta) ** 2 - np.trace(beta @ beta))
+                nut = c * math.sqrt(B / np.sum(G * G))
+                for (a, b), nm in zip(((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)), got):
+                    assert got[nm][i, j, k] == pytest.approx(-2 * nut * S[a, b], rel=1e-12, abs=0)
+    # hand-computed cell: alpha = diag(a1, a2, a3) gives B = sum over pairs of
+    # (D_m a_m D_n a_n)^2
+    s.set("U1", 1.5 * X)
+    s.set("U2", -0.5 * Y)
+    s.set("U3", -1.0 * Z)
+    s.lib.IBMSolver_calc_tau_r(s.h)
+    i, j, k = 1, 0, 2
+    d = [dx[i + 2] * 1.5, dy[j + 2] * 0.5, dz[k + 2] * 1.0]
+    B = (d[0] * d[1]) ** 2 + (d[0] * d[2]) ** 2 + (d[1] * d[2]) ** 2
+    nut = c * math.sqrt(B / (1.5 ** 2 + 0.5 ** 2 + 1.0))
+    assert s.get("TAU11")[i + 2, j + 2, k + 2] == pytest.approx(-2 * nut * 1.5, rel=1e-13)
+    assert s.get("TAU22")[i + 2, j + 2, k + 2] == pytest.approx(-2 * nut * -0.5, rel=1e-13)
+    # pure shear u = a y (one non-zero alpha): B = 0, so Vreman switches off
+    s.set("U1", 3.0 * Y)
+    s.set("U2", 0 * Y)
+    s.set("U3", 0 * Y)
+    s.lib.IBMSolver_calc_tau_r(s.h)
+    for nm in got:
+        assert np.all(s.get(nm)[2:-2, 2:-2, 2:-2] == 0), nm
+
+
+# ---------------------------------------------------------------------------
+# a11: in-place ghost-cell update  <- src/ibm3d_fracstep.c:1915-1976
+# ---------------------------------------------------------------------------
+def plane_body(I, s, cases, normal, a, n=(12, 10, 9)):
+    """A box cut by the plane normal . x = a (|normal| = 1, the body on the
+    side normal . x < a), level set installed directly: phi = normal . x - a
+    is the signed distance, so the ghost stencils' mirror points are exact."""
+    xf = cases.uniform_faces(0.0, 1.2, n[0])
+    yf = 0.5 * (1 - np.cos(np.pi * np.arange(n[1] + 1) / n[1]))
+    zf = cases.uniform_faces(0.0, 0.9, n[2])
+    box(I, s, xf, yf, zf, Re=10.0, dt=0.01)
+    lib = bind_oracle_hooks(s.lib)
+    lib.oracle_set_lvset.argtypes = [C.c_void_p, _dp]
+    xc, yc, zc, *_ = coords(s)
+    X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+    nv = np.asarray(normal, dtype=float) / np.linalg.norm(normal)
+    phi = np.ascontiguousarray(nv[0] * X + nv[1] * Y + nv[2] * Z - a)
+    lib.oracle_set_lvset(s.h, _p(phi))
+    return X, Y, Z, phi, nv
+
+
+PLANES = [((1.0, 0.3, 0.2), 0.45), ((-1.0, 2.0, -2.0), -0.6), ((-3.0, -1.0, 0.5), -1.0)]
+
+
+@pytest.mark.parametrize("normal,a", PLANES)
+def test_update_ghost_exact_about_a_tilted_plane(I, new_oracle, cases, normal, a):
+    """Mirror-image boundary conditions are exact for linear fields about a
+    plane: a field odd about it (u = c phi) gives u_g = -u(mirror) = c phi_g,
+    a field even about it (p linear along the plane) gives p_g = p(mirror) =
+    p(x_g). The exact ghost values are a fixed point of the pass, and from
+    zero ghost values repeated passes converge to them (the self weight on the
+    left-hand side with the sign of each condition: 1 + w for u, 1 - w for
+    p). Solid cells become NaN."""
+    s = new_oracle()
+    X, Y, Z, phi, nv = plane_body(I, s, cases, normal, a)
+    fl = s.flags()
+    ghost, solid = fl == I.FLAG_GHOST, fl == I.FLAG_SOLID
+    assert ghost[2:-2, 2:-2, 2:-2].sum() > 40 and solid[2:-2, 2:-2, 2:-2].sum() > 40
+    t = np.cross(nv, [0.3, -0.2, 1.0])  # tangential direction
+    exact = {"U1": 2.0 * phi, "U2": -0.7 * phi, "U3": 1.3 * phi,
+             "P": 0.25 + t[0] * X + t[1] * Y + t[2] * Z}
+    for nm, v in exact.items():
+        s.set(nm, v)
+    s.lib.oracle_update_ghost(s.h)
+    inner = np.zeros(fl.shape, bool)
+    inner[2:-2, 2:-2, 2:-2] = True
+    for nm, v in exact.items():
+        got = s.get(nm)
+        np.testing.assert_allclose(got[ghost & inner], v[ghost & inner], rtol=0, atol=1e-13)
+        assert np.isnan(got[solid & inner]).all()
+    # from zero ghost values
+    for nm, v in exact.items():
+        start = v.copy()
+        start[ghost & inner] = 0.0
+        s.set(nm, start)
+    for _ in range(40):
+        s.lib.oracle_update_ghost(s.h)
+    for nm, v in exact.items():
+        np.testing.assert_allclose(s.get(nm)[ghost & inner], v[ghost & inner], rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("normal,a", PLANES)
+def test_update_ghost_in_place_lexicographic_order(I, new_oracle, cases, normal, a):
+    """One pass from arbitrary ghost values equals a serial sweep over the
+    interior in (i, j, k) order, k fastest, in which a ghost stencil reads the
+    values already written to ghost cells earlier in that order and the old
+    values of later ones; solid stencil points are dropped and the weights
+    renormalised (src/ibm3d_fracstep.c:1915-1976). Brute force on the CPU with
+    the exported stencil routine (pinned by the mirror-point test). A
+    simultaneous (Jacobi) update differs on these planes, so the check sees
+    the order."""
+    s = new_oracle()
+    plane_body(I, s, cases, normal, a)
+    fl = s.flags()
+    rng = np.random.default_rng(11)
+    vals = {nm: rng.uniform(-1, 1, s.shape) for nm in ("U1", "U2", "U3", "P")}
+    for nm, v in vals.items():
+        s.set(nm, v)
+    s.lib.oracle_update_ghost(s.h)
+    got = {nm: s.get(nm) for nm in vals}
+
+    idx = np.zeros((8, 3), dtype=np.int32)
+    w = np.zeros(8)
+    seq = {nm: v.copy() for nm, v in vals.items()}
+    jac = {nm: v.copy() for nm, v in vals.items()}
+    for (i, j, k) in s.ghost_cells():  # lexicographic, k fastest
+        s.lib.IBMSolver_ghost_interp(s.h, int(i), int(j), int(k), idx.ctypes.data_as(_ip), _p(w))
+        keep = np.array([fl[a + 2, b + 2, c + 2] != I.FLAG_SOLID for a, b, c in idx])
+        wn = np.where(keep, w, 0.0) / w[keep].sum()
+        me = np.array([(a, b, c) == (i, j, k) for a, b, c in idx])
+        ws = wn[me].sum()
+        others = [(wn[l], tuple(idx[l] + 2)) for l in range(8) if keep[l] and not me[l]]
+        for out, src in ((seq, seq), (jac, vals)):
+            for nm in vals:
+                acc = sum(wl * src[nm][q] for wl, q in others)
+                if nm == "P":
+                    if 1 - ws != 0:
+                        out[nm][i + 2, j + 2, k + 2] = acc / (1 - ws)
+                else:
+                    out[nm][i + 2, j + 2, k + 2] = -acc / (1 + ws)
+    g = fl == I.FLAG_GHOST
+    differs = False
+    for nm in vals:
+        np.testing.assert_allclose(got[nm][g], seq[nm][g], rtol=1e-13, atol=1e-14)
+        differs |= not np.allclose(got[nm][g], jac[nm][g], rtol=1e-10, atol=1e-12)
+    # where the body lies on the high side of an axis, mirror stencils reach
+    # ghost cells that come earlier in the sweep
+    assert differs == (min(normal) < 0)
