@@ -202,7 +202,10 @@ def sphere_faces(nx: int = 256, ny: int = 128, nz: int = 128, dmin: float = SPHE
 def setup_sphere(s: I.Solver, nx: int = 256, ny: int = 128, nz: int = 128, Re: float = 300.0,
                  dt: float = 0.002, tol: float = 1e-10, precision: int = I.PREC_FP64,
                  dmin: float | None = None, faces=None, solver: int = I.SOLVER_BiCGSTAB,
-                 precond: int = I.PRECOND_AMG):
+                 precond: int = I.PRECOND_AMG, turb: tuple | None = None):
+    """Flow past the analytic sphere phi = |x| - 0.5: velocity inlet (1, 0, 0)
+    on the West face, pressure outlet 0 on the East face, free-slip lateral
+    walls; turb = (model, coefficient) switches on an eddy-viscosity model."""
     if faces is None:
         if dmin is None:
             # keep the cell-width ratio of the 256x128x128 grid on coarser grids
@@ -215,6 +218,8 @@ def setup_sphere(s: I.Solver, nx: int = 256, ny: int = 128, nz: int = 128, Re: f
     s.set_bc(I.DIR_EAST, I.BC_PRESSURE, I.VAL_CONST, 0.0)
     s.set_bc(I.DIR_NORTH | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP, I.BC_FREE_SLIP_WALL)
     s.set_obstacle_sphere(0.0, 0.0, 0.0, 0.5)
+    if turb is not None:
+        s.set_turb_model(*turb)
     s.set_linear_solver(solver, precond, tol)
     s.set_precision(precision)
     s.assemble()

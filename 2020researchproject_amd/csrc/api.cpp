@@ -46,30 +46,20 @@ IBMSolver *IBMSolver_new(const int num_process, const int rank, const int Px, co
     if (num_process != 1 || rank != 0 || Px * Py * Pz != 1) {
         fprintf(stderr, "error: one GPU holds the whole domain: num_process must be 1 (got %d, %dx%dx%d)\n",
                 num_process, Px, Py, Pz);
+        fflush(stdout);
         abort();
     }
-    HipGlobalLock lk(hip_global_mutex());
-    IBMSolver *s = new IBMSolver();
-    DevBytesScope account(&s->dev_bytes);
-    HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    HIP_CHECK(hipHostMalloc((void **)&s->ks_h, sizeof(KScal), hipHostMallocDefault));
-    HIP_CHECK(hipHostMalloc((void **)&s->pub_h, sizeof(KPub), hipHostMallocCoherent | hipHostMallocMapped));
-    memset((void *)s->pub_h, 0, sizeof(KPub));
-    HIP_CHECK(hipHostGetDevicePointer((void **)&s->pub_d, s->pub_h, 0));
-    HIP_CHECK(hipHostMalloc((void **)&s->sums_h, (kSumSlots + 1) * sizeof(double),
-                            hipHostMallocCoherent | hipHostMallocMapped));
-    memset((void *)s->sums_h, 0, (kSumSlots + 1) * sizeof(double));
-    HIP_CHECK(hipHostGetDevicePointer((void **)&s->sums_d, s->sums_h, 0));
-    s->sums_h_epoch = reinterpret_cast<unsigned *>(s->sums_h + kSumSlots);
-    s->sums_d_epoch = reinterpret_cast<unsigned *>(s->sums_d + kSumSlots);
-    HIP_CHECK(hipHostMalloc((void **)&s->gm_h, 64 * sizeof(double), hipHostMallocDefault));
-    s->ks.alloc(1);
-    s->sums.alloc(kSumSlots);
-    return s;
+    // the device resources are created by IBMSolver_assemble once its input
+    // checks pass (init_device), so a rejected setup never touches the GPU
+    return new IBMSolver();
 }
 
 void IBMSolver_destroy(IBMSolver *s) {
     if (!s) return;
+    if (!s->stream) {  // never assembled: no device resources
+        delete s;
+        return;
+    }
     (void)hipStreamSynchronize(s->stream);
     HipGlobalLock lk(hip_global_mutex());
     for (auto &t : s->timers)
@@ -620,29 +610,73 @@ static void alloc_fields(IBMSolver *s) {
 
 }  // namespace ibm
 
+// Stream, mapped host slots for Krylov scalars and step sums, and the device
+// scalars of one solver.
+static void init_device(IBMSolver *s) {
+    HipGlobalLock lk(hip_global_mutex());
+    HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIP_CHECK(hipHostMalloc((void **)&s->ks_h, sizeof(KScal), hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc((void **)&s->pub_h, sizeof(KPub), hipHostMallocCoherent | hipHostMallocMapped));
+    memset((void *)s->pub_h, 0, sizeof(KPub));
+    HIP_CHECK(hipHostGetDevicePointer((void **)&s->pub_d, s->pub_h, 0));
+    HIP_CHECK(hipHostMalloc((void **)&s->sums_h, (kSumSlots + 1) * sizeof(double),
+                            hipHostMallocCoherent | hipHostMallocMapped));
+    memset((void *)s->sums_h, 0, (kSumSlots + 1) * sizeof(double));
+    HIP_CHECK(hipHostGetDevicePointer((void **)&s->sums_d, s->sums_h, 0));
+    s->sums_h_epoch = reinterpret_cast<unsigned *>(s->sums_h + kSumSlots);
+    s->sums_d_epoch = reinterpret_cast<unsigned *>(s->sums_d + kSumSlots);
+    HIP_CHECK(hipHostMalloc((void **)&s->gm_h, 64 * sizeof(double), hipHostMallocDefault));
+    s->ks.alloc(1);
+    s->sums.alloc(kSumSlots);
+}
+
 extern "C" void IBMSolver_assemble(IBMSolver *s) {
     if (s->Nx == 0) fatal("IBMSolver_set_grid must precede IBMSolver_assemble");
-    DevBytesScope account(&s->dev_bytes);
+    if (s->assembled) fatal("IBMSolver_assemble called twice");
     // <- src/ibm3d_setup.c:523-533
     if (isperiodic(s->bc[0].type) != isperiodic(s->bc[2].type) || isperiodic(s->bc[1].type) != isperiodic(s->bc[3].type)
         || isperiodic(s->bc[4].type) != isperiodic(s->bc[5].type)) {
         printf("Inconsistent periodic boundary condition\n");
+        fflush(stdout);
         abort();
     }
     // pressure solver and preconditioner types  <- src/ibm3d_setup.c:1435-1536
     if (s->solver_type != SOLVER_AMG && s->solver_type != SOLVER_PCG && s->solver_type != SOLVER_BiCGSTAB &&
         s->solver_type != SOLVER_GMRES) {
         printf("\nUnknown linear solver type\n");
+        fflush(stdout);
         abort();
     }
     if (s->precond_type != PRECOND_NONE && s->precond_type != PRECOND_AMG) {
         printf("\nUnknown preconditioner type\n");
+        fflush(stdout);
         abort();
     }
     if (s->solver_type == SOLVER_AMG && s->precond_type != PRECOND_NONE) {
         printf("\nCannot use preconditioner with BoomerAMG solver\n");
+        fflush(stdout);
         abort();
     }
+    // IBM3D_PREC_MIXED runs BiCGSTAB defect correction for every system
+    if (s->precision == IBM3D_PREC_MIXED && s->solver_type != SOLVER_BiCGSTAB)
+        fatal("IBM3D_PREC_MIXED supports only SOLVER_BiCGSTAB for the pressure (use IBM3D_PREC_MIXED_KRYLOV "
+              "or IBM3D_PREC_FP64 for GMRES, PCG and AMG)");
+    // Red-black colouring of the multigrid smoother: on a periodic axis with
+    // an odd cell count the two cells beside the wrap share a colour, and a
+    // colour launch would read a neighbour it is writing.
+    if (s->precond_type == PRECOND_AMG || s->solver_type == SOLVER_AMG) {
+        const int n[3] = {s->Nx, s->Ny, s->Nz};
+        const char *ax = "xyz";
+        for (int a = 0; a < 3; a++)
+            if (isperiodic(s->bc[face_bc(a, 0)].type) && n[a] % 2) {
+                fprintf(stderr, "error: periodic %c axis with an odd cell count (%d): the multigrid smoother's "
+                        "red-black colouring needs an even count (or use PRECOND_NONE)\n", ax[a], n[a]);
+                fflush(stdout);
+                abort();
+            }
+    }
+    DevBytesScope account(&s->dev_bytes);
+    init_device(s);
     s->p_singular = true;  // <- src/ibm3d_setup.c:1598-1603
     for (int i = 0; i < 6; i++)
         if (s->bc[i].type == BC_PRESSURE || s->bc[i].type == BC_VELOCITY_PERIODIC

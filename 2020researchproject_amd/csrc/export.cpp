@@ -50,45 +50,64 @@ void append_be(std::vector<char> &out, const T *v, size_t n) {
 
 int type_size(int t) { return t == NC_DOUBLE_T ? 8 : t == NC_CHAR_T ? 1 : 4; }
 
+// Classic format with 32-bit offsets (CDF-1), or with 64-bit offsets (CDF-2)
+// when the file would pass the 2 GiB that CDF-1 offsets address. Returns
+// false (nothing written) when the file cannot be opened or a variable is
+// too large for the format.
 bool write_cdf1(const std::string &path, const std::vector<std::pair<std::string, int>> &dims,
                 const std::vector<NcVar> &vars) {
-    std::vector<char> h;
-    h.insert(h.end(), {'C', 'D', 'F', 1});
-    put_u32(h, 0);  // numrecs: no record dimension
-    put_u32(h, 0x0A);
-    put_u32(h, (uint32_t)dims.size());
-    for (auto &d : dims) { put_name(h, d.first); put_u32(h, (uint32_t)d.second); }
-    put_u32(h, 0); put_u32(h, 0);  // no global attributes
-    put_u32(h, 0x0B);
-    put_u32(h, (uint32_t)vars.size());
-    std::vector<size_t> begin_pos;
+    size_t payload = 0;
     for (auto &v : vars) {
-        put_name(h, v.name);
-        put_u32(h, (uint32_t)v.dims.size());
-        for (int d : v.dims) put_u32(h, (uint32_t)d);
-        if (v.atts.empty()) { put_u32(h, 0); put_u32(h, 0); }
-        else {
-            put_u32(h, 0x0C);
-            put_u32(h, (uint32_t)v.atts.size());
-            for (auto &a : v.atts) {
-                put_name(h, a.first);
-                put_u32(h, NC_CHAR_T);
-                put_u32(h, (uint32_t)a.second.size());
-                h.insert(h.end(), a.second.begin(), a.second.end());
-                while (h.size() % 4) h.push_back(0);
-            }
+        if (v.data.size() > 0xFFFFFFFCull) {
+            fprintf(stderr, "error: variable %s (%zu bytes) exceeds the netCDF classic variable size\n",
+                    v.name.c_str(), v.data.size());
+            return false;
         }
-        put_u32(h, (uint32_t)v.type);
-        size_t vsize = v.data.size();
-        vsize = (vsize + 3) / 4 * 4;
-        put_u32(h, (uint32_t)vsize);
-        begin_pos.push_back(h.size());
-        put_u32(h, 0);  // begin, patched below
+        payload += (v.data.size() + 3) / 4 * 4;
     }
+    // header size does not depend on the offset values, only on their width
+    auto header = [&](int version, std::vector<size_t> *begin_pos) {
+        std::vector<char> h;
+        h.insert(h.end(), {'C', 'D', 'F', (char)version});
+        put_u32(h, 0);  // numrecs: no record dimension
+        put_u32(h, 0x0A);
+        put_u32(h, (uint32_t)dims.size());
+        for (auto &d : dims) { put_name(h, d.first); put_u32(h, (uint32_t)d.second); }
+        put_u32(h, 0); put_u32(h, 0);  // no global attributes
+        put_u32(h, 0x0B);
+        put_u32(h, (uint32_t)vars.size());
+        for (auto &v : vars) {
+            put_name(h, v.name);
+            put_u32(h, (uint32_t)v.dims.size());
+            for (int d : v.dims) put_u32(h, (uint32_t)d);
+            if (v.atts.empty()) { put_u32(h, 0); put_u32(h, 0); }
+            else {
+                put_u32(h, 0x0C);
+                put_u32(h, (uint32_t)v.atts.size());
+                for (auto &a : v.atts) {
+                    put_name(h, a.first);
+                    put_u32(h, NC_CHAR_T);
+                    put_u32(h, (uint32_t)a.second.size());
+                    h.insert(h.end(), a.second.begin(), a.second.end());
+                    while (h.size() % 4) h.push_back(0);
+                }
+            }
+            put_u32(h, (uint32_t)v.type);
+            put_u32(h, (uint32_t)((v.data.size() + 3) / 4 * 4));
+            begin_pos->push_back(h.size());
+            put_u32(h, 0);  // begin, patched below
+            if (version == 2) put_u32(h, 0);
+        }
+        return h;
+    };
+    std::vector<size_t> begin_pos;
+    std::vector<char> h = header(1, &begin_pos);
+    const int version = h.size() + payload > 0x7FFFFFFFull ? 2 : 1;
+    if (version == 2) { begin_pos.clear(); h = header(2, &begin_pos); }
     size_t off = h.size();
     for (size_t q = 0; q < vars.size(); q++) {
-        const uint32_t b = (uint32_t)off;
-        for (int s = 0; s < 4; s++) h[begin_pos[q] + s] = (char)((b >> (24 - 8 * s)) & 0xff);
+        const int w = version == 2 ? 8 : 4;
+        for (int s = 0; s < w; s++) h[begin_pos[q] + s] = (char)((off >> (8 * (w - 1 - s))) & 0xff);
         off += (vars[q].data.size() + 3) / 4 * 4;
     }
     FILE *f = fopen(path.c_str(), "wb");
@@ -121,7 +140,9 @@ NcVar var_i(const std::string &name, std::vector<int> dims, const int *v, size_t
     return x;
 }
 
-// minimal CDF-1/CDF-2 reader: variable name -> (type, raw big-endian bytes)
+// minimal CDF-1/CDF-2 reader: variable name -> (type, dimension lengths,
+// raw big-endian bytes). Every read is bounded by the buffer; a malformed or
+// truncated header makes parse() return false.
 struct NcFile {
     std::vector<char> buf;
     struct V { int type; size_t begin, vsize; std::vector<int> dims; };
@@ -129,7 +150,13 @@ struct NcFile {
     std::vector<int> dimlen;
     size_t pos = 0;
     int version = 1;
+    bool bad = false;
+    bool have(size_t n) {
+        if (pos > buf.size() || buf.size() - pos < n) bad = true;
+        return !bad;
+    }
     uint32_t u32() {
+        if (!have(4)) return 0;
         uint32_t v = 0;
         for (int s = 0; s < 4; s++) v = (v << 8) | (unsigned char)buf[pos++];
         return v;
@@ -137,51 +164,64 @@ struct NcFile {
     uint64_t u64() { uint64_t a = u32(); return (a << 32) | u32(); }
     std::string name() {
         const uint32_t n = u32();
+        const size_t padded = ((size_t)n + 3) / 4 * 4;
+        if (!have(padded)) return std::string();
         std::string s(buf.data() + pos, n);
-        pos += (n + 3) / 4 * 4;
+        pos += padded;
         return s;
     }
     bool parse() {
         if (buf.size() < 32 || buf[0] != 'C' || buf[1] != 'D' || buf[2] != 'F') return false;
         version = buf[3];
+        if (version != 1 && version != 2) return false;
         pos = 4;
         u32();  // numrecs
         uint32_t tag = u32(), n = u32();
         if (tag == 0x0A)
-            for (uint32_t q = 0; q < n; q++) { name(); dimlen.push_back((int)u32()); }
+            for (uint32_t q = 0; q < n && !bad; q++) { name(); dimlen.push_back((int)u32()); }
         tag = u32(); n = u32();
         if (tag == 0x0C) skip_atts(n);
         tag = u32(); n = u32();
+        if (bad) return false;
         if (tag != 0x0B) return n == 0;
-        for (uint32_t q = 0; q < n; q++) {
+        for (uint32_t q = 0; q < n && !bad; q++) {
             V v;
             const std::string nm = name();
             const uint32_t nd = u32();
-            for (uint32_t d = 0; d < nd; d++) v.dims.push_back((int)u32());
+            for (uint32_t d = 0; d < nd && !bad; d++) {
+                const uint32_t id = u32();
+                if (id >= dimlen.size()) bad = true;
+                else v.dims.push_back(dimlen[id]);
+            }
             const uint32_t at = u32(), an = u32();
             if (at == 0x0C) skip_atts(an);
             v.type = (int)u32();
             v.vsize = u32();
             v.begin = version == 2 ? (size_t)u64() : (size_t)u32();
+            if (v.begin > buf.size() || buf.size() - v.begin < v.vsize) bad = true;
             vars.push_back({nm, v});
         }
-        return true;
+        return !bad;
     }
     void skip_atts(uint32_t n) {
-        for (uint32_t a = 0; a < n; a++) {
+        for (uint32_t a = 0; a < n && !bad; a++) {
             name();
             const int t = (int)u32();
             const uint32_t cnt = u32();
-            pos += ((size_t)cnt * type_size(t) + 3) / 4 * 4;
+            const size_t len = ((size_t)cnt * type_size(t) + 3) / 4 * 4;
+            if (have(len)) pos += len;
         }
     }
     const V *find(const std::string &nm) const {
         for (auto &v : vars) if (v.first == nm) return &v.second;
         return nullptr;
     }
+    // n values of v (the caller checked v's type and shape; parse() checked
+    // that its vsize bytes lie inside the buffer)
     template <typename T>
     void read(const V &v, T *out, size_t n) const {
         const size_t w = sizeof(T);
+        if (n * w > v.vsize) fatal("netCDF variable shorter than its shape");
         for (size_t i = 0; i < n; i++) {
             unsigned char tmp[8];
             for (size_t k = 0; k < w; k++) tmp[k] = (unsigned char)buf[v.begin + i * w + w - 1 - k];
@@ -212,6 +252,7 @@ std::string with_ext(const char *filename) { return std::string(filename) + ".nc
 
 // <- src/ibm3d_export.c:16-253
 void export_result_impl(IBMSolver *s, const char *filename) {
+    if (!s->assembled) fatal("IBMSolver_assemble must precede IBMSolver_export_*");
     const int X = s->Nx + 4, Y = s->Ny + 4, Z = s->Nz + 4;
     const std::vector<float> u = to_zyx(s, fetch_ref(s, IBM3D_FIELD_U1));
     const std::vector<float> v = to_zyx(s, fetch_ref(s, IBM3D_FIELD_U2));
@@ -264,6 +305,7 @@ void export_result_impl(IBMSolver *s, const char *filename) {
 
 // <- src/ibm3d_export.c:262-435
 void export_lvset_flag_impl(IBMSolver *s, const char *filename) {
+    if (!s->assembled) fatal("IBMSolver_assemble must precede IBMSolver_export_*");
     const int X = s->Nx + 4, Y = s->Ny + 4, Z = s->Nz + 4;
     const size_t n = (size_t)X * Y * Z;
     const std::vector<float> lv = to_zyx(s, s->lvset_h);
@@ -284,6 +326,7 @@ void export_lvset_flag_impl(IBMSolver *s, const char *filename) {
 
 // <- src/ibm3d_export.c:437-650 (u~ formed from u* and p on cells -1..N)
 void export_intermediate_impl(IBMSolver *s, const char *filename) {
+    if (!s->assembled) fatal("IBMSolver_assemble must precede IBMSolver_export_*");
     const int X = s->Nx + 4, Y = s->Ny + 4, Z = s->Nz + 4;
     const size_t n = (size_t)X * Y * Z;
     std::vector<double> us[3] = {fetch_ref(s, IBM3D_FIELD_U1_STAR), fetch_ref(s, IBM3D_FIELD_U2_STAR),
@@ -321,10 +364,11 @@ void export_intermediate_impl(IBMSolver *s, const char *filename) {
         fprintf(stderr, "error: cannot open file %s\n", with_ext(filename).c_str());
 }
 
-// <- src/ibm3d_init.c:45-224: time, iter, u, v, w, p from "<filename>.nc"
+// <- src/ibm3d_init.c:45-224: time, iter, u, v, w, p from "<filename>.nc".
+// The file is opened, parsed and checked (types, and the padded grid shape of
+// the solver's IBMSolver_set_grid) before anything else, so a missing,
+// truncated or mismatched file aborts with its own message.
 void init_flow_file_impl(IBMSolver *s, const char *filename) {
-    if (!s->assembled) fatal("IBMSolver_assemble must precede IBMSolver_init_flow_file");
-    s->conv_ready = false;
     NcFile nc;
     FILE *f = fopen(with_ext(filename).c_str(), "rb");
     if (!f) {
@@ -334,22 +378,35 @@ void init_flow_file_impl(IBMSolver *s, const char *filename) {
     fseek(f, 0, SEEK_END);
     const long len = ftell(f);
     fseek(f, 0, SEEK_SET);
-    nc.buf.resize(len);
-    if (fread(nc.buf.data(), 1, len, f) != (size_t)len) { fclose(f); fatal("cannot read restart file"); }
+    nc.buf.resize(len > 0 ? len : 0);
+    if (len <= 0 || fread(nc.buf.data(), 1, len, f) != (size_t)len) { fclose(f); fatal("cannot read restart file"); }
     fclose(f);
-    if (!nc.parse()) fatal("restart file is not netCDF classic");
+    if (!nc.parse()) fatal("restart file is not netCDF classic, or is truncated");
     const char *need[6] = {"time", "iter", "u", "v", "w", "p"};
     for (const char *nm : need)
         if (!nc.find(nm)) {
-            fprintf(stderr, "error: variable '%s' not found\n", nm);
+            fprintf(stderr, "error: varible '%s' not found\n", nm);
             abort();
         }
-    nc.read(*nc.find("time"), &s->time, 1);
-    nc.read(*nc.find("iter"), &s->iter, 1);
     const int X = s->Nx + 4, Y = s->Ny + 4, Z = s->Nz + 4;
-    const size_t n = (size_t)X * Y * Z;
-    if (nc.find("u")->vsize < n * 4) fatal("restart grid does not match the solver grid");
+    const NcFile::V &vt = *nc.find("time"), &vi = *nc.find("iter");
+    if (vt.type != NC_DOUBLE_T || vi.type != NC_INT_T) fatal("restart file: time must be double and iter int");
+    const std::vector<int> shape = {1, Z, Y, X};
     const char *fields[4] = {"u", "v", "w", "p"};
+    for (const char *nm : fields) {
+        const NcFile::V &v = *nc.find(nm);
+        if (v.type != NC_FLOAT_T || v.dims != shape) {
+            fprintf(stderr, "error: restart variable '%s' must be float [1][%d][%d][%d] (the solver's padded grid)\n",
+                    nm, Z, Y, X);
+            abort();
+        }
+    }
+    // the file is checked in full before the solver is touched
+    if (!s->assembled) fatal("IBMSolver_assemble must precede IBMSolver_init_flow_file");
+    s->conv_ready = false;
+    nc.read(vt, &s->time, 1);
+    nc.read(vi, &s->iter, 1);
+    const size_t n = (size_t)X * Y * Z;
     const int ids[4] = {IBM3D_FIELD_U1, IBM3D_FIELD_U2, IBM3D_FIELD_U3, IBM3D_FIELD_P};
     std::vector<float> buf(n);
     std::vector<double> ref(n);

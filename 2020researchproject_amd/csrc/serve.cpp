@@ -141,21 +141,34 @@ void IBMServer_destroy(IBMServer *sv) {
 double IBMSolver_estimate_device_bytes(int Nx, int Ny, int Nz, int precision, int les) {
     const ibm::Dims d = ibm::make_dims(Nx, Ny, Nz);
     const double cells = (double)d.ncell;
-    // step fields (u, u*, U, U*, N, N_prev, b per component; p, p', b_p,
-    // p_next; level set), Krylov vectors, residual scratch; flags
+    const bool mixed = precision != IBM3D_PREC_FP64;
+    // per padded cell, as allocated by alloc_fields: step fields (u, u_next,
+    // u*, U, U_next, U*, N, N_prev, b per component; p, p_next, p', b_p),
+    // level set, 8 Krylov vectors, flags
     double per = 8.0 * (27 + 4 + 1 + 8) + 1;
-    if (les) per += 8.0 * 6;
-    if (precision != IBM3D_PREC_FP64) per += 8 + 4 * 2;
+    if (les) per += 8.0 * 6;  // tau
+    // the two extra lanes of the concurrent momentum solves: 8 FP64 vectors
+    // each, plus 2 FP32 preconditioned directions each in mode 2
+    if (precision != IBM3D_PREC_MIXED) per += 2 * 8.0 * 8;
+    if (precision == IBM3D_PREC_MIXED_KRYLOV) per += 2 * 2 * 4.0;
+    // refinement vectors: FP64 residual, FP32 right-hand side and correction
+    // (+ a second FP32 correction in mode 2); mode 1: 8 FP32 Krylov vectors
+    if (mixed) per += 8 + 4 + 4 + (precision == IBM3D_PREC_MIXED_KRYLOV ? 4 : 0);
     if (precision == IBM3D_PREC_MIXED) per += 4 * 8;
-    // multigrid: fine-level scratch (2 arrays) and coarse levels (x, b, r,
-    // second smoother buffer, flags and face masks) in FP64 (+ FP32); the
-    // coarse levels hold at most 1.5 times the fine cells at gamma 3
-    const double lvl = 8.0 * 4 + 2 + (precision != IBM3D_PREC_FP64 ? 4.0 * 4 : 0);
-    double mg = cells * (8.0 * 2 + (precision != IBM3D_PREC_FP64 ? 8 : 0)) + 1.5 * cells * lvl;
-    // dense inverse of the coarsest level (at most 3000 rows)
+    // multigrid (build_mg): fine-level residual and smoother scratch (FP64,
+    // + FP32); coarse levels x, b, r, second smoother buffer (FP64, + FP32),
+    // flag and face-mask bytes, and one interpolation-mask byte per cell of
+    // the next finer level. The coarse levels hold at most 1.5 times the fine
+    // cells at the default coarsening threshold.
+    const double coarse = 1.5 * cells;
+    const double mg = cells * (8.0 * 2 + (mixed ? 4.0 * 2 : 0))
+        + coarse * (8.0 * 4 + 2 + (mixed ? 4.0 * 4 : 0)) + (cells + coarse) * 1;
+    // dense inverse of the coarsest level (at most 3000 rows; FP64, + FP32)
+    // and the pivot scratch of its inversion
     const double n = std::min(3000.0, (double)Nx * Ny * Nz);
-    const double dense = n * n * (8 + (precision != IBM3D_PREC_FP64 ? 4 : 0));
-    return cells * per + mg + dense + 8.0 * (1 << 20);
+    const double dense = n * n * (8 + (mixed ? 4 : 0)) + 2 * 8.0 * n;
+    // reduction partials, ghost lists, 1-D metrics and folded coefficients
+    return cells * per + mg + dense + 16.0 * (1 << 20);
 }
 
 double IBMSolver_get_device_bytes(IBMSolver *s) { return (double)s->dev_bytes; }

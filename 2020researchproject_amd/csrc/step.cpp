@@ -1584,7 +1584,10 @@ int IBMSolver_get_kernel_stat(IBMSolver *s, int idx, char *name, int namelen, do
     return (int)s->timers.size();
 }
 
-void IBMSolver_calc_tau_r(IBMSolver *s) { calc_tau(s); }
+void IBMSolver_calc_tau_r(IBMSolver *s) {
+    if (!s->assembled) fatal("IBMSolver_assemble must precede IBMSolver_calc_tau_r");
+    calc_tau(s);
+}
 
 int IBMSolver_get_step_times(IBMSolver *s, double *out, int max) {
     const int n = (int)s->step_ms.size();

@@ -46,9 +46,12 @@ double IBMServer_peak_bytes(IBMServer *);
 /* Waits for the queued cases, then joins the workers. */
 void IBMServer_destroy(IBMServer *);
 
-/* Upper estimate of the device memory a case allocates (fields, multigrid
-   hierarchy, coarsest dense inverse) for an Nx x Ny x Nz grid; les: 1 if a
-   turbulence model is on. */
+/* Upper estimate of the device memory a case allocates (fields, Krylov and
+   refinement vectors, the concurrent momentum lanes, multigrid hierarchy,
+   coarsest dense inverse) for an Nx x Ny x Nz grid with the BiCGSTAB, PCG or
+   AMG pressure solver; les: 1 if a turbulence model is on. Not counted:
+   SOLVER_GMRES's basis (21 more FP64 vectors per cell) and function-valued
+   forcing (3 FP64 arrays per cell). */
 double IBMSolver_estimate_device_bytes(int Nx, int Ny, int Nz, int precision, int les);
 /* Device memory the solver allocated during IBMSolver_new / assemble. */
 double IBMSolver_get_device_bytes(IBMSolver *);

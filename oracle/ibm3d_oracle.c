@@ -31,6 +31,7 @@ static const int ADJ[6][3] = {
 
 static void fatal(const char *msg) {
     fprintf(stderr, "error: %s\n", msg);
+    fflush(stdout);
     abort();
 }
 
@@ -416,6 +417,25 @@ void IBMSolver_assemble(IBMSolver *s) {
         isperiodic(s->bc[1].type) != isperiodic(s->bc[3].type) ||
         isperiodic(s->bc[4].type) != isperiodic(s->bc[5].type)) {
         printf("Inconsistent periodic boundary condition\n");  /* :523-533 */
+        fflush(stdout);
+        abort();
+    }
+    /* solver / preconditioner type checks, src/ibm3d_setup.c:1435-1536 (the
+       oracle itself always solves with BiCGSTAB) */
+    if (s->solver_type != SOLVER_AMG && s->solver_type != SOLVER_PCG && s->solver_type != SOLVER_BiCGSTAB
+        && s->solver_type != SOLVER_GMRES) {
+        printf("\nUnknown linear solver type\n");
+        fflush(stdout);
+        abort();
+    }
+    if (s->precond_type != PRECOND_NONE && s->precond_type != PRECOND_AMG) {
+        printf("\nUnknown preconditioner type\n");
+        fflush(stdout);
+        abort();
+    }
+    if (s->solver_type == SOLVER_AMG && s->precond_type != PRECOND_NONE) {
+        printf("\nCannot use preconditioner with BoomerAMG solver\n");
+        fflush(stdout);
         abort();
     }
     /* ghost widths and centres, :536-588 */
@@ -1112,7 +1132,10 @@ static void calc_u_star(IBMSolver *s) {
             }
         double norm;
         const int it = oracle_bicgstab(&s->A[l - 1], s->b, s->x, s->tol, 200, 0, s->work, &norm);
-        if (!isfinite(norm)) fatal("floating point error in u_star");
+        if (!isfinite(norm)) {  /* src/ibm3d_fracstep.c:363-366 */
+            fprintf(stderr, "error: floating pointer error raised in u%d_star\n", l);
+            abort();
+        }
         if (norm > s->tol) fprintf(stderr, "warning: u%d_star did not converge\n", l);
         memcpy(us[l], s->x, sizeof(double) * P);
         s->final_norm[l - 1] = norm;
@@ -1222,7 +1245,10 @@ static void calc_p_prime(IBMSolver *s) {
     double norm;
     const int use_ilu = s->precond_type == PRECOND_AMG && s->A[3].has_ilu;
     const int it = oracle_bicgstab(&s->A[3], s->b, s->x, s->tol, 1000, use_ilu, s->work, &norm);
-    if (!isfinite(norm)) fatal("floating point error in p_prime");
+    if (!isfinite(norm)) {  /* src/ibm3d_fracstep.c:673-676 */
+        fprintf(stderr, "error: floating pointer error raised in p_prime\n");
+        abort();
+    }
     if (norm > s->tol) fprintf(stderr, "warning: p_prime did not converge\n");
     memcpy(s->p_prime, s->x, sizeof(double) * P);
     s->final_norm[3] = norm;

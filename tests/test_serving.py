@@ -50,3 +50,20 @@ def test_serving_same_result_alone_and_in_mix(I):
         m = next(r for r in mix["results"] if r.spec is target).fields
         for n in ("U1", "U2", "U3", "P"):
             assert np.array_equal(a[n], m[n], equal_nan=True), (target, n)
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+@pytest.mark.parametrize("les", [0, 1])
+def test_footprint_estimate_bounds_allocation_at_full_size(new_gpu, cases, I, precision, les):
+    """Admission relies on IBMSolver_estimate_device_bytes being an upper
+    bound of what a case allocates: checked on the 256x128x128 sphere (where
+    the per-cell terms dominate the fixed ones) in each precision mode, with
+    and without the eddy-viscosity fields, and not loose by more than 30 %."""
+    s = new_gpu()
+    cases.setup_sphere(s, 256, 128, 128, precision=precision,
+                       turb=(I.TURBMODEL_SMAGORINSKY, 0.1) if les else None)
+    got = s.device_bytes()
+    est = s.lib.IBMSolver_estimate_device_bytes(256, 128, 128, precision, les)
+    print(f"precision {precision} les {les}: allocated {got / 1e9:.3f} GB, estimate {est / 1e9:.3f} GB",
+          flush=True)
+    assert got <= est <= 1.3 * got
