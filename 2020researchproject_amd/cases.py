@@ -202,10 +202,13 @@ def sphere_faces(nx: int = 256, ny: int = 128, nz: int = 128, dmin: float = SPHE
 def setup_sphere(s: I.Solver, nx: int = 256, ny: int = 128, nz: int = 128, Re: float = 300.0,
                  dt: float = 0.002, tol: float = 1e-10, precision: int = I.PREC_FP64,
                  dmin: float | None = None, faces=None, solver: int = I.SOLVER_BiCGSTAB,
-                 precond: int = I.PRECOND_AMG, turb: tuple | None = None):
+                 precond: int = I.PRECOND_AMG, turb: tuple | None = None, outflow: str = "pressure"):
     """Flow past the analytic sphere phi = |x| - 0.5: velocity inlet (1, 0, 0)
-    on the West face, pressure outlet 0 on the East face, free-slip lateral
-    walls; turb = (model, coefficient) switches on an eddy-viscosity model."""
+    on the West face, free-slip lateral walls, and on the East face a
+    pressure outlet p = 0 (outflow "pressure", the reference's BC_PRESSURE) or
+    a convective outflow advected with the mean outlet velocity ("convective",
+    BC_CONVECTIVE_OUTFLOW); turb = (model, coefficient) switches on an
+    eddy-viscosity model."""
     if faces is None:
         if dmin is None:
             # keep the cell-width ratio of the 256x128x128 grid on coarser grids
@@ -215,7 +218,12 @@ def setup_sphere(s: I.Solver, nx: int = 256, ny: int = 128, nz: int = 128, Re: f
     s.set_grid(xf, yf, zf)
     s.set_params(Re, dt)
     s.set_bc(I.DIR_WEST, I.BC_VELOCITY_COMPONENT, I.VAL_CONST, 1.0, 0.0, 0.0)
-    s.set_bc(I.DIR_EAST, I.BC_PRESSURE, I.VAL_CONST, 0.0)
+    if outflow == "convective":
+        s.set_bc(I.DIR_EAST, I.BC_CONVECTIVE_OUTFLOW, 0.0)
+    elif outflow == "pressure":
+        s.set_bc(I.DIR_EAST, I.BC_PRESSURE, I.VAL_CONST, 0.0)
+    else:
+        raise ValueError(f"unknown outflow {outflow!r}")
     s.set_bc(I.DIR_NORTH | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP, I.BC_FREE_SLIP_WALL)
     s.set_obstacle_sphere(0.0, 0.0, 0.0, 0.5)
     if turb is not None:

@@ -354,6 +354,16 @@ static void build_ghosts(IBMSolver *s) {
                     lex.push_back(g);
                 }
     const int ng = (int)lex.size();
+    {
+        std::vector<int> ijk;
+        ijk.reserve(3 * (size_t)ng + 3);
+        for (int i = 0; i < Nx; i++)
+            for (int j = 0; j < Ny; j++)
+                for (int k = 0; k < Nz; k++)
+                    if (s->flag_ref[host_ref_index(s, i, j, k)] == FLAG_GHOST) ijk.insert(ijk.end(), {i, j, k});
+        if (ijk.empty()) ijk.assign(3, 0);
+        s->ghost_ijk.upload(ijk);
+    }
     int gi = 0;
     std::vector<int> level(ng, 0);
     bool touches_outer = false;
@@ -901,37 +911,7 @@ extern "C" double IBMSolver_max_div(IBMSolver *s) { return max_div_impl(s); }
 extern "C" void IBMSolver_calc_body_force(IBMSolver *s, double out[3]) {
     out[0] = out[1] = out[2] = 0;
     if (!s->assembled) return;
-    const long P = (long)(s->Nx + 4) * (s->Ny + 4) * (s->Nz + 4);
-    std::vector<double> u[3], p(P);
-    for (int a = 0; a < 3; a++) { u[a].resize(P); IBMSolver_get_field(s, IBM3D_FIELD_U1 + a, u[a].data()); }
-    IBMSolver_get_field(s, IBM3D_FIELD_P, p.data());
-    const double mu = 1.0 / s->Re;
-    const std::vector<double> *D[3] = {&s->dx, &s->dy, &s->dz}, *C[3] = {&s->xc, &s->yc, &s->zc};
-    for (int i = 0; i < s->Nx; i++)
-        for (int j = 0; j < s->Ny; j++)
-            for (int k = 0; k < s->Nz; k++) {
-                const long q = host_ref_index(s, i, j, k);
-                if (s->flag_ref[q] != FLAG_FLUID) continue;
-                const int id[3] = {i, j, k};
-                for (int a = 0; a < 3; a++)
-                    for (int sd = -1; sd <= 1; sd += 2) {
-                        int nb[3] = {i, j, k};
-                        nb[a] += sd;
-                        const long g = host_ref_index(s, nb[0], nb[1], nb[2]);
-                        if (s->flag_ref[g] != FLAG_GHOST) continue;
-                        // face area and centre distance
-                        double area = 1;
-                        for (int b = 0; b < 3; b++) if (b != a) area *= H1((*D[b]), id[b]);
-                        const double h = std::fabs(H1((*C[a]), id[a]) - H1((*C[a]), nb[a]));
-                        const double pf = 0.5 * (p[q] + p[g]);
-                        // outward body normal is -sd along axis a
-                        out[a] += pf * sd * area;
-                        for (int b = 0; b < 3; b++) {
-                            if (b == a) continue;
-                            out[b] += mu * (u[b][q] - u[b][g]) / h * area;
-                        }
-                    }
-            }
+    body_force_read(s, out);
 }
 
 extern "C" void IBMSolver_iterate(IBMSolver *s, int steps, bool verbose) {

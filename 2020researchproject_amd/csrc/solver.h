@@ -267,7 +267,11 @@ struct _ibm_solver {
     std::vector<int> ghost_lex;           // lexicographic order -> slot
     std::vector<int> level_start;
     ibm::DBuf<ibm::GhostCell> gc;
+    ibm::DBuf<int> ghost_ijk;             // i, j, k of each ghost cell, lexicographic order
     ibm::DBuf<double> ghost_old;
+    // body force after each step of IBMSolver_iterate (IBMSolver_set_force_history)
+    ibm::DBuf<double> force_hist;         // [force_hist_max][3]
+    int force_hist_max = 0, force_hist_n = 0;
     ibm::GhostData gmom, gp;              // solver ghost rows (momentum / pressure)
     int ghost_prev = -1;                  // where the pressure ghost values are (launch_ghost_relax)
 
@@ -310,6 +314,8 @@ void build_system(const IBMSolver *s, const std::vector<double> *D[3], const std
 void iterate_impl(IBMSolver *s, int steps, bool verbose);
 void build_mg(IBMSolver *s);
 double max_div_impl(IBMSolver *s);
+void body_force_impl(IBMSolver *s, double *dst);
+void body_force_read(IBMSolver *s, double out[3]);
 
 // timing helpers (step.cpp)
 int timer_begin(IBMSolver *s, const char *name, double bytes);

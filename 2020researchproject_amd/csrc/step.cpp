@@ -1505,6 +1505,7 @@ void iterate_impl(IBMSolver *s, int steps, bool verbose) {
     // step times: events recorded on the stream at each step's end (the host
     // runs ahead into the next step instead of draining the stream)
     s->step_ms.clear();
+    s->force_hist_n = 0;
     std::vector<hipEvent_t> ends;
     hipEvent_t start = get_event(s);
     HIP_CHECK(hipEventRecord(start, s->stream));
@@ -1515,6 +1516,10 @@ void iterate_impl(IBMSolver *s, int steps, bool verbose) {
         s->time = t0 + s->dt * i;
         update_outer(s);
         update_ghost(s);
+        if (i <= s->force_hist_max) {  // body force of the step's fields, kept on the device
+            body_force_impl(s, s->force_hist.p + 3 * (i - 1));
+            s->force_hist_n = i;
+        }
         ends.push_back(get_event(s));
         HIP_CHECK(hipEventRecord(ends.back(), s->stream));
         if (verbose) {
@@ -1543,6 +1548,20 @@ void iterate_impl(IBMSolver *s, int steps, bool verbose) {
         prev = e;
     }
     s->event_pool.push_back(prev);
+}
+
+// Body force from the current u and p into dst[0..3) on the device (two
+// fixed-order reduction stages).
+void body_force_impl(IBMSolver *s, double *dst) {
+    const double *u[3] = {s->u[0].p, s->u[1].p, s->u[2].p};
+    launch_body_force(s->stream, s->geo, s->flag.p, s->ghost_ijk.p, s->ng, 1.0 / s->Re, s->p.p, u, s->partial.p);
+    launch_sum_partials(s->stream, s->partial.p, body_force_blocks(s->ng), 3, dst, 0);
+}
+
+void body_force_read(IBMSolver *s, double out[3]) {
+    body_force_impl(s, s->sums.p);
+    read_sums(s, 3);
+    for (int a = 0; a < 3; a++) out[a] = s->sums_h[a];
 }
 
 double max_div_impl(IBMSolver *s) {
@@ -1587,6 +1606,26 @@ int IBMSolver_get_kernel_stat(IBMSolver *s, int idx, char *name, int namelen, do
 void IBMSolver_calc_tau_r(IBMSolver *s) {
     if (!s->assembled) fatal("IBMSolver_assemble must precede IBMSolver_calc_tau_r");
     calc_tau(s);
+}
+
+void IBMSolver_set_force_history(IBMSolver *s, int max_steps) {
+    if (!s->assembled) fatal("IBMSolver_assemble must precede IBMSolver_set_force_history");
+    if (max_steps < 0) fatal("IBMSolver_set_force_history: max_steps must be >= 0");
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->force_hist = DBuf<double>();
+    if (max_steps > 0) s->force_hist.alloc(3 * (size_t)max_steps);
+    s->force_hist_max = max_steps;
+    s->force_hist_n = 0;
+}
+
+int IBMSolver_get_force_history(IBMSolver *s, double *out, int max) {
+    const int n = s->force_hist_n;
+    if (out && n > 0 && max > 0) {
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        HIP_CHECK(hipMemcpy(out, s->force_hist.p, 3 * (size_t)std::min(n, max) * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
+    return n;
 }
 
 int IBMSolver_get_step_times(IBMSolver *s, double *out, int max) {

@@ -85,6 +85,11 @@ void launch_solid_nan(hipStream_t st, const Dims &d, const uint8_t *flag, double
 void launch_sgs(hipStream_t st, const Geo &g, int model, double coef, const VanDriest &vd, const uint8_t *flag,
                 const double *lvset, const double *const u[3], const TauPtrs &tau);
 long maxdiv_blocks(const Dims &d);
+// body force from the faces between interior ghost cells (gijk: i, j, k of
+// each, ng of them) and their interior fluid neighbours: partial[blk * 3 + a]
+long body_force_blocks(int ng);
+void launch_body_force(hipStream_t st, const Geo &g, const uint8_t *flag, const int *gijk, int ng, double mu,
+                       const double *p, const double *const u[3], double *partial);
 void launch_maxdiv(hipStream_t st, const Geo &g, const uint8_t *flag, const double *const U[3], double *partial);
 
 }  // namespace ibm

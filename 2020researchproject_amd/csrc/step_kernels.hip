@@ -653,6 +653,50 @@ __global__ void __launch_bounds__(256) k_maxdiv(Geo g, const uint8_t *flag, cons
     }
 }
 
+// Force on the immersed body (pressure and viscous parts, per unit density)
+// from the staircase surface: the faces between an interior ghost cell and an
+// interior fluid 6-neighbour, one thread per ghost cell. With n the outward
+// body normal (from the ghost into the fluid cell) and A the face area:
+//   pressure: F -= p_f n A, p_f linearly interpolated to the face position;
+//   viscous:  F_b += mu (u_b(fluid) - u_b(ghost)) / |x_fluid - x_ghost| A for
+//             the two tangential components b.
+// No reference counterpart (the reference has no force diagnostic).
+__global__ void __launch_bounds__(256) k_body_force(Geo g, const uint8_t *flag, const int *gijk, int ng, double mu,
+                                                    const double *p, const double *u1, const double *u2,
+                                                    const double *u3, double *partial) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    double f[3] = {0, 0, 0};
+    if (t < ng) {
+        const Dims &d = g.d;
+        const int id[3] = {gijk[3 * t], gijk[3 * t + 1], gijk[3 * t + 2]};
+        const long q = cidx(d, id[0], id[1], id[2]);
+        const int n[3] = {d.nx, d.ny, d.nz};
+        const long st[3] = {1, d.sy, d.sz};
+        const double *C[3] = {g.xc, g.yc, g.zc}, *D[3] = {g.dx, g.dy, g.dz};
+        const double *u[3] = {u1, u2, u3};
+        const double h[3] = {C1(g.dx, id[0]), C1(g.dy, id[1]), C1(g.dz, id[2])};
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int sd = -1; sd <= 1; sd += 2) {
+                const int nb = id[a] + sd;
+                if (nb < 0 || nb >= n[a]) continue;
+                const long r = q + sd * st[a];
+                if (flag[r] != F_FLUID) continue;
+                const double area = h[0] * h[1] * h[2] / h[a];
+                const double xg = C1(C[a], id[a]), xf = C1(C[a], nb);
+                const double xface = xg + sd * 0.5 * C1(D[a], id[a]);
+                const double pf = p[q] + (p[r] - p[q]) * ((xface - xg) / (xf - xg));
+                f[a] -= pf * sd * area;
+                const double inv = 1.0 / fabs(xf - xg);
+#pragma unroll
+                for (int b = 0; b < 3; b++)
+                    if (b != a) f[b] += mu * (u[b][r] - u[b][q]) * inv * area;
+            }
+    }
+    block_sum_store<3>(f, partial, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -747,6 +791,14 @@ void launch_sgs(hipStream_t st, const Geo &g, int model, double coef, const VanD
 }
 
 long maxdiv_blocks(const Dims &d) { return conv_rhs_blocks(d); }
+
+long body_force_blocks(int ng) { return (ng + 255) / 256 > 0 ? (ng + 255) / 256 : 1; }
+
+void launch_body_force(hipStream_t st, const Geo &g, const uint8_t *flag, const int *gijk, int ng, double mu,
+                       const double *p, const double *const u[3], double *partial) {
+    hipLaunchKernelGGL(k_body_force, dim3(body_force_blocks(ng)), dim3(256), 0, st, g, flag, gijk, ng, mu, p, u[0],
+                       u[1], u[2], partial);
+}
 
 void launch_maxdiv(hipStream_t st, const Geo &g, const uint8_t *flag, const double *const U[3], double *partial) {
     hipLaunchKernelGGL(k_maxdiv, cell_grid(g.d, 0), dim3(64, 4), 0, st, g, flag, U[0], U[1], U[2], partial);

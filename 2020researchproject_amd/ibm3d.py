@@ -82,6 +82,8 @@ _SIGS = {
 # optional symbols (present in the product, absent in the oracle or vice versa)
 _OPTIONAL = {
     "IBMSolver_calc_body_force": (None, [_vp, _dp]),
+    "IBMSolver_set_force_history": (None, [_vp, C.c_int]),
+    "IBMSolver_get_force_history": (C.c_int, [_vp, _dp, C.c_int]),
     "IBMSolver_enable_timing": (None, [_vp, C.c_int]),
     "IBMSolver_set_graph_replay": (None, [_vp, C.c_int]),
     "IBMSolver_get_step_times": (C.c_int, [_vp, _dp, C.c_int]),
@@ -311,6 +313,16 @@ class Solver:
         return out
 
     # --- timing (product only) ----------------------------------------------
+    def set_force_history(self, max_steps: int):
+        self.lib.IBMSolver_set_force_history(self.h, int(max_steps))
+
+    def force_history(self) -> np.ndarray:
+        n = self.lib.IBMSolver_get_force_history(self.h, None, 0)
+        out = np.zeros((n, 3))
+        if n:
+            self.lib.IBMSolver_get_force_history(self.h, _dptr(out), n)
+        return out
+
     def enable_timing(self, mode: int = 1):
         """0 off, 1 every kernel (multigrid cycles eager), 2 kernels outside
         the cycles plus whole graph replays of the cycles."""

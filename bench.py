@@ -3,23 +3,28 @@
 
 Workload: flow past a sphere (phi = |x| - 0.5), domain [-5,15]x[-5,5]^2,
 256x128x128 cells cosine-stretched toward the body, Re = 300, dt = 0.002,
-velocity inlet / pressure outlet / free-slip walls, linear solves to relative
-residual 1e-9 (which keeps max|div U| <= 1e-8 for both precisions). A step
-is one IBMSolver_iterate time step (AB2 convection, three Crank-Nicolson
-Helmholtz solves, pressure Poisson solve, projection, boundary and
-immersed-boundary ghost updates). Inputs are resident in HBM when the timed
-region starts.
+uniform inflow u = 1, convective outflow, free-slip lateral walls, linear
+solves to relative residual 1e-9 (which keeps max|div U| <= 1e-8 for both
+precisions). A step is one IBMSolver_iterate time step (AB2 convection,
+three Crank-Nicolson Helmholtz solves, pressure Poisson solve, projection,
+boundary and immersed-boundary ghost updates, and the drag/lift reduction
+of the step's fields). Inputs are resident in HBM when the timed region
+starts.
 
 The headline value is the mixed path in Mcell-steps/s: FP64 BiCGSTAB whose
 multigrid preconditioner runs in FP32 (IBM3D_PREC_MIXED_KRYLOV). The plain
-FP64 path runs the same steps for the speed-up, max|div U| and velocity
-agreement. Prints one JSON line.
+FP64 path runs the same steps for the speed-up. After the timed regions both
+GPU paths and the FP64 CPU oracle run the first steps of the same case from
+the same initial field; their differences (rel-L2 and Linf of u and p,
+max|div U|, drag and lift) are the line's error statement. Prints one JSON
+line.
 """
 from __future__ import annotations
 
 import argparse
 import importlib
 import json
+import math
 import os
 import sys
 import time
@@ -33,6 +38,13 @@ I, cases = PKG.ibm3d, PKG.cases
 
 METRIC = "Mcell-steps/s & p99 latency: FP32-MG+FP64 refine vs FP64, same div tol"
 HBM_PEAK_GBS = 8000.0
+ORACLE_LIB = os.path.join(ROOT, "oracle", "libibm3d_oracle.so")
+HALF_AREA = 0.5 * math.pi * 0.25  # (1/2) U^2 A_ref, U = 1, A_ref = pi r^2 of the r = 0.5 sphere
+
+
+def coefficients(force):
+    """Drag and lift coefficients (Cd, Cl_y, Cl_z) of a body force."""
+    return [float(f) / HALF_AREA for f in force]
 
 
 def log(msg: str) -> None:
@@ -40,7 +52,7 @@ def log(msg: str) -> None:
 
 
 def run_gpu(prec: int, nx: int, ny: int, nz: int, steps: int, warmup: int, tol: float, prof_steps: int,
-            graphs: bool = True):
+            graphs: bool = True, outflow: str = "convective"):
     """Set up the sphere case, run `warmup` untimed steps, time `steps`
     steps (no per-kernel instrumentation: multigrid cycles replay as HIP
     graphs), then run `prof_steps` more steps with every kernel bracketed by
@@ -48,22 +60,26 @@ def run_gpu(prec: int, nx: int, ny: int, nz: int, steps: int, warmup: int, tol: 
     s = I.Solver()
     t0 = time.perf_counter()
     s.set_graph_replay(graphs)
-    cases.setup_sphere(s, nx, ny, nz, tol=tol, precision=prec)
+    cases.setup_sphere(s, nx, ny, nz, tol=tol, precision=prec, outflow=outflow)
     log(f"[bench] setup prec={prec} {nx}x{ny}x{nz}: {time.perf_counter() - t0:.2f} s, "
         f"ghost cells {len(s.ghost_cells())}")
     if warmup:
         s.iterate(warmup)
         log(f"[bench] warmup {warmup} steps done (its {s.last_iters().tolist()})")
+    s.set_force_history(steps)  # drag / lift of every timed step, kept on the device
     t0 = time.perf_counter()
     s.iterate(steps)
     wall = time.perf_counter() - t0
     step_ms = s.step_times()
+    force_hist = s.force_history()
+    s.set_force_history(0)
     res = {
         "wall": wall, "step_ms": step_ms, "max_div": s.max_div(),
         "iters": s.last_iters().tolist(), "norms": s.last_norms().tolist(),
         "u": [s.get(n)[2:-2, 2:-2, 2:-2] for n in ("U1", "U2", "U3")],
         "p": s.get("P")[2:-2, 2:-2, 2:-2],
         "force": s.body_force().tolist(),
+        "coef_hist": [coefficients(f) for f in force_hist],
         "mg_levels": ["x".join(str(v) for v in lv) for lv in s.mg_levels()],
     }
     log(f"[bench] prec={prec} timed {steps} steps: {wall:.3f} s, max|div U| {res['max_div']:.3e}, "
@@ -89,25 +105,120 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a[m] - b[m]) / max(np.linalg.norm(b[m]), 1e-300))
 
 
-def cpu_baseline(tol: float, steps: int = 3):
-    """Oracle (FP64 CPU restatement, OpenMP, ILU(0)-BiCGSTAB pressure) on a
-    bounded sample of the same workload: `steps` time steps of the sphere
-    case on the 128x64x64 grid (same domain, stretching law and physics,
-    1/8 of the cells; the full grid takes minutes per step on the CPU)."""
-    lib = os.path.join(ROOT, "oracle", "libibm3d_oracle.so")
-    if not os.path.exists(lib):
-        return None
-    nx, ny, nz = 128, 64, 64
-    o = I.Solver(lib)
-    cases.setup_sphere(o, nx, ny, nz, tol=tol)
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+
+
+def fields(s):
+    u = [s.get(n)[2:-2, 2:-2, 2:-2] for n in ("U1", "U2", "U3")]
+    return u, s.get("P")[2:-2, 2:-2, 2:-2]
+
+
+def errors(u, p, ref_u, ref_p):
+    """rel-L2 and Linf (relative to the largest reference value) of the
+    velocity as a vector field and of p, over the cells where both are finite
+    (NaN marks solid cells in both)."""
+    m = np.isfinite(ref_p) & np.isfinite(p)
+    for a, b in zip(u, ref_u):
+        m &= np.isfinite(a) & np.isfinite(b)
+    du = np.sqrt(sum((a[m] - b[m]) ** 2 for a, b in zip(u, ref_u)))
+    su = np.sqrt(sum(b[m] ** 2 for b in ref_u))
+    dp = p[m] - ref_p[m]
+    return {"rel_l2_u": float(np.linalg.norm(du) / max(np.linalg.norm(su), 1e-300)),
+            "linf_u": float(du.max() / max(su.max(), 1e-300)),
+            "rel_l2_p": float(np.linalg.norm(dp) / max(np.linalg.norm(ref_p[m]), 1e-300)),
+            "linf_p": float(np.abs(dp).max() / max(np.abs(ref_p[m]).max(), 1e-300))}
+
+
+def vs_oracle(nx, ny, nz, tol, steps, outflow, mixed_mode):
+    """Both GPU paths and the FP64 CPU oracle (all host threads) run `steps`
+    steps of the benchmark case from the same initial field. Returns the
+    error statement and the oracle's timing (the CPU baseline on the full
+    workload)."""
+    o = I.Solver(ORACLE_LIB)
+    cases.setup_sphere(o, nx, ny, nz, tol=tol, outflow=outflow)
+    log(f"[bench] oracle {nx}x{ny}x{nz} assembled; {steps} steps on {host_threads()} host threads")
     t0 = time.perf_counter()
     o.iterate(steps)
-    dt = time.perf_counter() - t0
+    t_oracle = time.perf_counter() - t0
+    log(f"[bench] oracle {steps} steps: {t_oracle:.1f} s, its {o.last_iters().tolist()}")
+    ref_u, ref_p = fields(o)
+    ref = {"max_div": o.max_div(), "coef": coefficients(o.body_force()), "iters_last_step": o.last_iters().tolist()}
+    ref_flags, ref_ghosts = o.flags(), o.ghost_cells()
     o.close()
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": nx * ny * nz * steps / dt / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} steps of the sphere case on {nx}x{ny}x{nz} (1/8 of the cells), "
-                      f"FP64 oracle, {dt:.2f} s"}
+    out = {"steps": steps, "grid": f"{nx}x{ny}x{nz}", "oracle": ref}
+    for tag, prec in (("mixed", mixed_mode), ("fp64", I.PREC_FP64)):
+        g = I.Solver()
+        cases.setup_sphere(g, nx, ny, nz, tol=tol, precision=prec, outflow=outflow)
+        if tag == "mixed":
+            out["flags_bit_exact"] = bool(np.array_equal(g.flags(), ref_flags))
+            out["ghost_cells_bit_exact"] = bool(np.array_equal(g.ghost_cells(), ref_ghosts))
+        g.iterate(steps)
+        u, p = fields(g)
+        e = errors(u, p, ref_u, ref_p)
+        e["max_div"] = g.max_div()
+        e["coef"] = coefficients(g.body_force())
+        e["coef_abs_err"] = max(abs(a - b) for a, b in zip(e["coef"], ref["coef"]))
+        out[tag] = e
+        g.close()
+        log(f"[bench] {tag} vs oracle: {e}")
+    cb = {"value": nx * ny * nz * steps / t_oracle / 1e6, "unit": "Mcell-steps/s", "cores": host_threads(),
+          "kind": "port", "cpu_model": cpu_model(),
+          "sample": f"the first {steps} steps of the same sphere case at full size ({nx}x{ny}x{nz}), FP64 "
+                    f"oracle (OpenMP, block-ILU(0) BiCGSTAB pressure), {t_oracle:.1f} s"}
+    return out, cb
+
+
+def cavity_full(tol: float = 1e-10, steps: int = 200):
+    """The 32^3 lid-driven cavity (Re 100, dt 0.005, seeded perturbation) run
+    in full on the CPU oracle (all host threads) and on the GPU in FP64; the
+    oracle's rate and the GPU's deviation from it. Also the oracle on one
+    thread for a 20-step sample of the same case."""
+    o = I.Solver(ORACLE_LIB)
+    cases.setup_cavity(o, n=32, tol=tol)
+    t0 = time.perf_counter()
+    o.iterate(steps)
+    t_full = time.perf_counter() - t0
+    ref_u, ref_p = fields(o)
+    o.close()
+    g = I.Solver()
+    cases.setup_cavity(g, n=32, tol=tol)
+    g.iterate(steps)
+    u, p = fields(g)
+    g.close()
+    e = errors(u, p, ref_u, ref_p)
+    # the pressure of the cavity is defined up to the pinned cell; both pin p'(0,0,0)
+    import ctypes
+    gomp = ctypes.CDLL("libgomp.so.1")
+    n1 = 20
+    o = I.Solver(ORACLE_LIB)
+    cases.setup_cavity(o, n=32, tol=tol)
+    gomp.omp_set_num_threads(1)
+    try:
+        t0 = time.perf_counter()
+        o.iterate(n1)
+        t_one = time.perf_counter() - t0
+    finally:
+        gomp.omp_set_num_threads(host_threads())
+    o.close()
+    cells = 32 ** 3
+    log(f"[bench] cavity 32^3: oracle {steps} steps {t_full:.1f} s, one thread {n1} steps {t_one:.1f} s; "
+        f"GPU FP64 vs oracle {e}")
+    return {"workload": f"cavity 32^3, Re 100, dt 0.005, {steps} steps, tol {tol:g}",
+            "value": round(cells * steps / t_full / 1e6, 4), "seconds": round(t_full, 2),
+            "gpu_fp64_vs_oracle": e,
+            "one_thread": {"value": round(cells * n1 / t_one / 1e6, 4), "sample": f"the same case, {n1} steps"}}
 
 
 def measured_traffic(kernel: str):
@@ -219,6 +330,11 @@ def main() -> int:
     ap.add_argument("--prof-steps", type=int, default=2, help="instrumented steps for the per-kernel table")
     ap.add_argument("--stats", action="store_true", help="print per-kernel timing to stderr")
     ap.add_argument("--no-graphs", action="store_true", help="launch multigrid cycles kernel by kernel")
+    ap.add_argument("--outflow", choices=("convective", "pressure"), default="convective",
+                    help="East boundary: convective outflow (the workload as stated) or the reference's "
+                         "pressure outlet")
+    ap.add_argument("--oracle-steps", type=int, default=2,
+                    help="steps of the full-size comparison with the FP64 CPU oracle (0: skip)")
     ap.add_argument("--serving", action="store_true",
                     help="run the serving stream (64 cavity/sphere cases, Poisson arrivals) instead")
     ap.add_argument("--serving-cases", type=int, default=64)
@@ -237,14 +353,15 @@ def main() -> int:
     if args.copy_calib_mb > 0:
         copy_calibration(args.copy_calib_mb)
     mixed = run_gpu(args.mixed_mode, nx, ny, nz, args.steps, args.warmup, args.tol, args.prof_steps,
-                    not args.no_graphs)
+                    not args.no_graphs, args.outflow)
     value = cells * args.steps / mixed["wall"] / 1e6
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(mixed["wall"] / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"sphere {nx}x{ny}x{nz} cosine-stretched, Re 300, dt 0.002, tol {args.tol:g}",
+        "config": {"workload": f"sphere {nx}x{ny}x{nz} cosine-stretched, Re 300, dt 0.002, uniform inflow, "
+                               f"{args.outflow} outflow, slip lateral walls, tol {args.tol:g}",
                    "cells": int(cells),
                    "precision": ("FP64 BiCGSTAB, FP32 multigrid preconditioner" if args.mixed_mode == 2 else
                                  "FP64 defect correction around FP32 multigrid-BiCGSTAB"),
@@ -253,6 +370,9 @@ def main() -> int:
         "p99_ms": round(float(np.percentile(mixed["step_ms"], 99)), 3),
         "max_div": mixed["max_div"],
         "iters_last_step": mixed["iters"],
+        # drag and lift coefficients (Cd, Cl_y, Cl_z) after each timed step
+        "cd_history": [round(c[0], 6) for c in mixed["coef_hist"]],
+        "cl_history": [[round(c[1], 6), round(c[2], 6)] for c in mixed["coef_hist"]],
     }
     rf = roofline(mixed["stats"])
     if rf:
@@ -262,7 +382,7 @@ def main() -> int:
         print_stats("mixed", mixed)
     if not args.no_fp64:
         f64 = run_gpu(I.PREC_FP64, nx, ny, nz, args.steps, args.warmup, args.tol, args.prof_steps if args.stats else 0,
-                      not args.no_graphs)
+                      not args.no_graphs, args.outflow)
         v64 = cells * args.steps / f64["wall"] / 1e6
         out["fp64_value"] = round(v64, 2)
         out["speedup_vs_fp64"] = round(value / v64, 3)
@@ -270,11 +390,14 @@ def main() -> int:
         out["fp64_p99_ms"] = round(float(np.percentile(f64["step_ms"], 99)), 3)
         out["rel_l2_u_mixed_vs_fp64"] = max(rel_l2(a, b) for a, b in zip(mixed["u"], f64["u"]))
         out["fp64_iters_last_step"] = f64["iters"]
+        out["cd_abs_diff_mixed_vs_fp64"] = max(abs(a[0] - b[0]) for a, b in zip(mixed["coef_hist"], f64["coef_hist"]))
         if args.stats:
             print_stats("fp64", f64)
-    if not args.no_cpu:
-        cb = cpu_baseline(args.tol)
-        if cb:
+    if args.oracle_steps > 0 and os.path.exists(ORACLE_LIB):
+        acc, cb = vs_oracle(nx, ny, nz, args.tol, args.oracle_steps, args.outflow, args.mixed_mode)
+        out["vs_oracle"] = acc
+        if not args.no_cpu:
+            cb["c1_cavity_full"] = cavity_full()
             out["cpu_baseline"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cb.items()}
     print(json.dumps(out), flush=True)
     return 0

@@ -92,9 +92,25 @@ int IBMSolver_get_mg_levels(IBMSolver *, int *out, int max);
 /* max |div U| over fluid cells, U the face velocity after projection. */
 double IBMSolver_max_div(IBMSolver *);
 
-/* Pressure + viscous force on the immersed body from the ghost-cell surface
-   (Fx, Fy, Fz); drag and lift coefficients follow by scaling. */
+/* Force on the immersed body per unit density (Fx, Fy, Fz), pressure plus
+   viscous part, summed over the staircase surface: the faces between an
+   interior ghost cell and an interior fluid 6-neighbour. With n the outward
+   body normal (ghost -> fluid) and A the face area, each face adds
+   -p_f n A (p_f: p linearly interpolated between the two cell centres to the
+   face position) and, for the two tangential components b,
+   (1/Re) (u_b(fluid) - u_b(ghost)) / |x_fluid - x_ghost| A. Computed on the
+   GPU (two fixed-order reduction stages). Drag and lift coefficients follow
+   by scaling with 1 / (U^2 A_ref / 2). */
 void IBMSolver_calc_body_force(IBMSolver *, double out[3]);
+
+/* Records the body force (as IBMSolver_calc_body_force) after each of the
+   first max_steps steps of every IBMSolver_iterate call, on the device with
+   no host synchronisation; 0 switches it off. After IBMSolver_assemble. */
+void IBMSolver_set_force_history(IBMSolver *, int max_steps);
+/* Copies the recorded forces of the last IBMSolver_iterate call, 3 values per
+   step, up to max steps into out (when out != NULL); returns the number of
+   steps recorded. */
+int IBMSolver_get_force_history(IBMSolver *, double *out, int max);
 
 /* Milliseconds of each time step of the last IBMSolver_iterate call: the
    time between the completions of consecutive steps on the device (HIP

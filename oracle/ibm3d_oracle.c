@@ -812,13 +812,31 @@ static void spmv(const EllMat *A, const double *x, double *y) {
     }
 }
 
-/* ILU(0) on the ELL pattern, rows sorted by column. */
+/* Block-Jacobi ILU(0) for large systems: the rows split into contiguous
+   blocks and each block's ILU(0) keeps only the couplings inside the block,
+   so the blocks factor and solve independently (in parallel). Systems below
+   ILU_BLOCK_ROWS rows keep one block (plain ILU(0)). The block count depends
+   only on the row count, so the preconditioner, and with it every iterate,
+   does not depend on the number of threads. Only iteration counts depend on
+   the preconditioner (it stands in for the reference's BoomerAMG). */
+#define ILU_MAX_BLOCKS 16
+#define ILU_BLOCK_ROWS 262144
+
+static int ilu_blocks(int n) {
+    const int b = n / ILU_BLOCK_ROWS;
+    return b < 1 ? 1 : b > ILU_MAX_BLOCKS ? ILU_MAX_BLOCKS : b;
+}
+
+static int ilu_block_len(int n) { return (n + ilu_blocks(n) - 1) / ilu_blocks(n); }
+
 void oracle_ilu0_build(EllMat *A) {
     const int n = A->nrows;
+    const int bl = ilu_block_len(n);
     free(A->scol); free(A->lu); free(A->diagpos);
     A->scol = malloc((size_t)9 * n * sizeof(int));
     A->lu = malloc((size_t)9 * n * sizeof(double));
     A->diagpos = malloc((size_t)n * sizeof(int));
+    #pragma omp parallel for schedule(static)
     for (int r = 0; r < n; r++) {
         const size_t o = (size_t)9 * r;
         const int m = A->nnz[r];
@@ -831,19 +849,24 @@ void oracle_ilu0_build(EllMat *A) {
         A->diagpos[r] = -1;
         for (int l = 0; l < m; l++) if (A->scol[o + l] == r) A->diagpos[r] = l;
     }
-    for (int r = 0; r < n; r++) {
-        const size_t o = (size_t)9 * r;
-        const int m = A->nnz[r];
-        for (int a = 0; a < m && A->scol[o + a] < r; a++) {
-            const int kk = A->scol[o + a];
-            const size_t ok = (size_t)9 * kk;
-            double piv = A->diagpos[kk] >= 0 ? A->lu[ok + A->diagpos[kk]] : 0;
-            if (fabs(piv) < 1e-300) piv = 1;
-            A->lu[o + a] /= piv;
-            for (int b = a + 1; b < m; b++) {
-                const int jc = A->scol[o + b];
-                for (int c = 0; c < A->nnz[kk]; c++)
-                    if (A->scol[ok + c] == jc) { A->lu[o + b] -= A->lu[o + a] * A->lu[ok + c]; break; }
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int blk = 0; blk < ilu_blocks(n); blk++) {
+        const int lo = blk * bl, hi = lo + bl < n ? lo + bl : n;
+        for (int r = lo; r < hi; r++) {
+            const size_t o = (size_t)9 * r;
+            const int m = A->nnz[r];
+            for (int a = 0; a < m && A->scol[o + a] < r; a++) {
+                const int kk = A->scol[o + a];
+                if (kk < lo) continue;  /* coupling to another block: dropped */
+                const size_t ok = (size_t)9 * kk;
+                double piv = A->diagpos[kk] >= 0 ? A->lu[ok + A->diagpos[kk]] : 0;
+                if (fabs(piv) < 1e-300) piv = 1;
+                A->lu[o + a] /= piv;
+                for (int b = a + 1; b < m; b++) {
+                    const int jc = A->scol[o + b];
+                    for (int c = 0; c < A->nnz[kk]; c++)
+                        if (A->scol[ok + c] == jc) { A->lu[o + b] -= A->lu[o + a] * A->lu[ok + c]; break; }
+                }
             }
         }
     }
@@ -852,19 +875,26 @@ void oracle_ilu0_build(EllMat *A) {
 
 static void ilu_apply(const EllMat *A, const double *rhs, double *z) {
     const int n = A->nrows;
-    for (int r = 0; r < n; r++) {  /* L z = rhs (unit lower) */
-        const size_t o = (size_t)9 * r;
-        double acc = rhs[r];
-        for (int a = 0; a < A->nnz[r] && A->scol[o + a] < r; a++) acc -= A->lu[o + a] * z[A->scol[o + a]];
-        z[r] = acc;
-    }
-    for (int r = n - 1; r >= 0; r--) {  /* U z = z */
-        const size_t o = (size_t)9 * r;
-        double acc = z[r];
-        for (int a = 0; a < A->nnz[r]; a++) if (A->scol[o + a] > r) acc -= A->lu[o + a] * z[A->scol[o + a]];
-        double piv = A->diagpos[r] >= 0 ? A->lu[o + A->diagpos[r]] : 0;
-        if (fabs(piv) < 1e-300) piv = 1;
-        z[r] = acc / piv;
+    const int bl = ilu_block_len(n);
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int blk = 0; blk < ilu_blocks(n); blk++) {
+        const int lo = blk * bl, hi = lo + bl < n ? lo + bl : n;
+        for (int r = lo; r < hi; r++) {  /* L z = rhs (unit lower) */
+            const size_t o = (size_t)9 * r;
+            double acc = rhs[r];
+            for (int a = 0; a < A->nnz[r] && A->scol[o + a] < r; a++)
+                if (A->scol[o + a] >= lo) acc -= A->lu[o + a] * z[A->scol[o + a]];
+            z[r] = acc;
+        }
+        for (int r = hi - 1; r >= lo; r--) {  /* U z = z */
+            const size_t o = (size_t)9 * r;
+            double acc = z[r];
+            for (int a = 0; a < A->nnz[r]; a++)
+                if (A->scol[o + a] > r && A->scol[o + a] < hi) acc -= A->lu[o + a] * z[A->scol[o + a]];
+            double piv = A->diagpos[r] >= 0 ? A->lu[o + A->diagpos[r]] : 0;
+            if (fabs(piv) < 1e-300) piv = 1;
+            z[r] = acc / piv;
+        }
     }
 }
 
@@ -1625,6 +1655,44 @@ double IBMSolver_max_div(IBMSolver *s) {
                 if (fabs(d) > m) m = fabs(d);
             }
     return m;
+}
+
+/* Force on the immersed body per unit density, from the staircase surface
+   (faces between an interior ghost cell and an interior fluid 6-neighbour):
+   -p_f n A with p_f interpolated linearly to the face position, plus
+   (1/Re) (u_b(fluid) - u_b(ghost)) / |x_fluid - x_ghost| A for the tangential
+   components (n: outward body normal, ghost -> fluid). An extension with no
+   reference counterpart; ghost cells in lexicographic order. */
+void IBMSolver_calc_body_force(IBMSolver *s, double out[3]) {
+    const int n[3] = {s->Nx, s->Ny, s->Nz};
+    const double *C[3] = {s->xc, s->yc, s->zc}, *D[3] = {s->dx, s->dy, s->dz};
+    const double *u[3] = {s->u1, s->u2, s->u3};
+    const double mu = 1 / s->Re;
+    out[0] = out[1] = out[2] = 0;
+    for (int i = 0; i < n[0]; i++)
+        for (int j = 0; j < n[1]; j++)
+            for (int k = 0; k < n[2]; k++) {
+                if (C3(s, s->flag, i, j, k) != FLAG_GHOST) continue;
+                const int id[3] = {i, j, k};
+                for (int a = 0; a < 3; a++)
+                    for (int sd = -1; sd <= 1; sd += 2) {
+                        int f[3] = {i, j, k};
+                        f[a] += sd;
+                        if (f[a] < 0 || f[a] >= n[a]) continue;
+                        if (C3(s, s->flag, f[0], f[1], f[2]) != FLAG_FLUID) continue;
+                        double area = 1;
+                        for (int b = 0; b < 3; b++)
+                            if (b != a) area *= C1(D[b], id[b]);
+                        const double xg = C1(C[a], id[a]), xf = C1(C[a], f[a]);
+                        const double xface = xg + sd * C1(D[a], id[a]) / 2;
+                        const double pg = C3(s, s->p, i, j, k), pf = C3(s, s->p, f[0], f[1], f[2]);
+                        out[a] -= (pg + (pf - pg) * (xface - xg) / (xf - xg)) * sd * area;
+                        for (int b = 0; b < 3; b++)
+                            if (b != a)
+                                out[b] += mu * (C3(s, u[b], f[0], f[1], f[2]) - C3(s, u[b], i, j, k))
+                                    / fabs(xf - xg) * area;
+                    }
+            }
 }
 
 /* Applies system `type` (1..4) of the assembled reference matrix to x. Test

@@ -10,7 +10,7 @@ import sys
 
 from conftest import ROOT
 
-HEADERS = ["ibm3d.h", "ibm3d_ext.h", "geo3d.h"]
+HEADERS = ["ibm3d.h", "ibm3d_ext.h", "geo3d.h", "ibm3d_serve.h"]
 
 
 def declared_functions():
@@ -18,7 +18,7 @@ def declared_functions():
     for h in HEADERS:
         src = open(os.path.join(ROOT, "include", h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b((?:IBMSolver|Polyhedron|Vector)_\w+)\s*\(", src, flags=re.M):
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b((?:IBMSolver|IBMServer|Polyhedron|Vector)_\w+)\s*\(", src, flags=re.M):
             names.add(m.group(1))
     return names
 

@@ -607,3 +607,56 @@ def test_update_ghost_in_place_lexicographic_order(I, new_oracle, cases, normal,
     # where the body lies on the high side of an axis, mirror stencils reach
     # ghost cells that come earlier in the sweep
     assert differs == (min(normal) < 0)
+
+
+# ---------------------------------------------------------------------------
+# drag / lift: force on the body from the staircase ghost-cell surface
+# (extension, no reference counterpart)
+# ---------------------------------------------------------------------------
+def body_force_closed_forms(I, s, cases, check):
+    """Closed forms of the staircase surface integral. The faces between
+    interior fluid and ghost cells bound the union B of the non-fluid cells,
+    so by the divergence theorem on B (a union of boxes):
+      p = g . x (any grid; the face pressure is interpolated to the face):
+        F = -g |B|;
+      u_b = S x_a^2 / 2 on a uniform grid (the centre difference across a face
+        is the derivative at the face): F = (1/Re) S |B| e_b;
+      u_b = S x_a: F = 0.
+    `s` is a fresh solver; `check(got, expect)` compares."""
+    Re = 50.0
+    # stretched grid: pressure part
+    xf, yf, zf = cases.sphere_faces(24, 20, 20, 0.12)
+    cases.setup_sphere(s[0], faces=(xf, yf, zf), Re=Re)
+    fl = s[0].flags()[2:-2, 2:-2, 2:-2]
+    d = [np.diff(f) for f in (xf, yf, zf)]
+    vol = (d[0][:, None, None] * d[1][None, :, None] * d[2][None, None, :])[fl != I.FLAG_FLUID].sum()
+    assert (fl[[0, -1], :, :] == I.FLAG_FLUID).all() and (fl[:, [0, -1], :] == I.FLAG_FLUID).all()
+    xc, yc, zc = (cases.cell_centres(f) for f in (xf, yf, zf))
+    X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+    g = np.array([0.7, -1.3, 0.4])
+    for nm in ("U1", "U2", "U3"):
+        s[0].set(nm, 0 * X)
+    s[0].set("P", 2.0 + g[0] * X + g[1] * Y + g[2] * Z)
+    check(s[0].body_force(), -g * vol)
+    # uniform grid: viscous part
+    f = cases.uniform_faces(-1.5, 1.5, 24)
+    cases.setup_sphere(s[1], faces=(f, f, f), Re=Re)
+    fl = s[1].flags()[2:-2, 2:-2, 2:-2]
+    vol = (fl != I.FLAG_FLUID).sum() * (3.0 / 24) ** 3
+    c = cases.cell_centres(f)
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    S = 1.7
+    s[1].set("P", 0 * X)
+    s[1].set("U1", 0 * X)
+    s[1].set("U3", 0 * X)
+    s[1].set("U2", S * X**2 / 2)  # u_y = S x^2 / 2: F = S |B| / Re e_y
+    check(s[1].body_force(), np.array([0.0, S * vol / Re, 0.0]))
+    s[1].set("U2", 0 * X)
+    s[1].set("U3", S * Y)  # uniform shear: no net force
+    check(s[1].body_force(), np.zeros(3))
+
+
+def test_body_force_closed_forms(I, new_oracle, cases):
+    def check(got, expect):
+        np.testing.assert_allclose(got, expect, rtol=1e-12, atol=1e-13)
+    body_force_closed_forms(I, [new_oracle(), new_oracle()], cases, check)
