@@ -179,6 +179,11 @@ void Polyhedron_cpt(const Polyhedron *poly, const int nx, const int ny, const in
     ibm::polyhedron_cpt_grid(poly, nx, ny, nz, x, y, z, f, maxd);
 }
 
+int Polyhedron_cpt_noalloc(const Polyhedron *poly, const int nx, const int ny, const int nz, const double *x,
+                           const double *y, const double *z, double *f, const double maxd) {
+    return ibm::polyhedron_cpt_grid_noalloc(poly, nx, ny, nz, x, y, z, f, maxd);
+}
+
 }  // extern "C"
 
 namespace ibm {
@@ -407,6 +412,84 @@ void polyhedron_cpt_grid(const Polyhedron *poly, int nx, int ny, int nz, const d
         hipLaunchKernelGGL(k_cpt_fill, dim3((lines[ax] + 255) / 256), dim3(256), 0, 0, nx, ny, nz, df.p, maxd, ax);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpy(f, df.p, n * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+// ---------------------------------------------------------------------------
+// Polyhedron_cpt_noalloc: the same transform with every device buffer (the
+// coordinates, the feature list, the grid of distances) placed in one
+// file-scope device array instead of allocations, for callers that cannot
+// allocate device memory. The array lives as long as the process on every GPU
+// the library runs on; each GPU has its own copy at its own address, found
+// once per device with hipGetSymbolAddress (the host-side name `scratch` is
+// not a device address and is never passed to a kernel or a copy).
+// ---------------------------------------------------------------------------
+constexpr size_t kCptScratchBytes = IBM3D_CPT_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kCptScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static unsigned char *g_scratch_dev[kMaxDevices];  // per device, set on first use there
+
+// Device address of `scratch` on the current device (looked up once per
+// device, with that device current), or null with a message on failure.
+static unsigned char *scratch_on_current_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        fprintf(stderr, "error: Polyhedron_cpt_noalloc: no usable current device\n");
+        return nullptr;
+    }
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            fprintf(stderr, "error: Polyhedron_cpt_noalloc: hipGetSymbolAddress failed on device %d: %s\n", dev,
+                    hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_dev[dev] = static_cast<unsigned char *>(ptr);
+    }
+    return g_scratch_dev[dev];
+}
+
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+int polyhedron_cpt_grid_noalloc(const Polyhedron *poly, int nx, int ny, int nz, const double *x, const double *y,
+                                const double *z, double *f, double maxd) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) {
+        fprintf(stderr, "error: Polyhedron_cpt_noalloc: empty grid %d x %d x %d\n", nx, ny, nz);
+        return -1;
+    }
+    const std::vector<CptFeature> feat = build_features(poly, maxd);
+    const long n = (long)nx * ny * nz;
+    const size_t ox = 0, oy = ox + round256(nx * sizeof(double)), oz = oy + round256(ny * sizeof(double));
+    const size_t of = oz + round256(nz * sizeof(double));
+    const size_t op = of + round256(std::max<size_t>(feat.size(), 1) * sizeof(CptFeature));
+    const size_t need = op + round256(n * sizeof(double));
+    if (need > kCptScratchBytes) {
+        fprintf(stderr,
+                "error: Polyhedron_cpt_noalloc: %ld points and %zu extrusion features need %zu bytes of device "
+                "scratch, more than its %zu\n",
+                n, feat.size(), need, kCptScratchBytes);
+        return -1;
+    }
+    // one caller at a time in this process: the scratch array is shared
+    HipGlobalLock lk(hip_global_mutex());
+    unsigned char *base = scratch_on_current_device();
+    if (!base) return -1;
+    double *dx = reinterpret_cast<double *>(base + ox), *dy = reinterpret_cast<double *>(base + oy);
+    double *dz = reinterpret_cast<double *>(base + oz), *df = reinterpret_cast<double *>(base + op);
+    CptFeature *dfeat = reinterpret_cast<CptFeature *>(base + of);
+    HIP_CHECK(hipMemcpy(dx, x, nx * sizeof(double), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dy, y, ny * sizeof(double), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dz, z, nz * sizeof(double), hipMemcpyHostToDevice));
+    if (!feat.empty()) HIP_CHECK(hipMemcpy(dfeat, feat.data(), feat.size() * sizeof(CptFeature), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_cpt, dim3((n + 255) / 256), dim3(256), 0, 0, nx, ny, nz, dx, dy, dz, dfeat,
+                       (int)feat.size(), df);
+    const long lines[3] = {(long)ny * nz, (long)nx * nz, (long)nx * ny};
+    for (int ax : {2, 1, 0})
+        hipLaunchKernelGGL(k_cpt_fill, dim3((lines[ax] + 255) / 256), dim3(256), 0, 0, nx, ny, nz, df, maxd, ax);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(f, df, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
 }
 
 }  // namespace ibm

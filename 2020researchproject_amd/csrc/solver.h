@@ -325,6 +325,8 @@ void timers_flush(IBMSolver *s);
 // geo3d.cpp
 void polyhedron_cpt_grid(const Polyhedron *poly, int nx, int ny, int nz, const double *x, const double *y,
                          const double *z, double *f, double maxd);
+int polyhedron_cpt_grid_noalloc(const Polyhedron *poly, int nx, int ny, int nz, const double *x, const double *y,
+                                const double *z, double *f, double maxd);
 
 // export.cpp
 void export_result_impl(IBMSolver *s, const char *filename);

@@ -63,6 +63,29 @@ void Polyhedron_cpt(
     const double maxd
 );
 
+/* Device scratch of Polyhedron_cpt_noalloc, per GPU: 64 MiB. */
+#define IBM3D_CPT_SCRATCH_BYTES (64u << 20)
+
+/* Polyhedron_cpt without any device-memory allocation (no counterpart in the
+   reference). For callers that cannot allocate device memory, e.g. a
+   long-running serving loop that keeps the allocator out of its path. Same
+   arguments and the same values of f, bit for bit, as Polyhedron_cpt. Every
+   device buffer of the transform (coordinates, extrusion features, the
+   distances) lives in one fixed device array of IBM3D_CPT_SCRATCH_BYTES per
+   GPU, held until the process ends; it keeps the last call's contents.
+   Returns 0, or -1 with a message on stderr (and f untouched) when the grid
+   and the polyhedron's features need more than that, e.g. more than about
+   8 million points. Calls from several threads of one process are
+   serialised; calls that overlap on one GPU from other processes use their
+   own copy. */
+int Polyhedron_cpt_noalloc(
+    const Polyhedron *poly,
+    const int nx, const int ny, const int nz,
+    const double *x, const double *y, const double *z,
+    double *f,
+    const double maxd
+);
+
 #ifdef __cplusplus
 }
 #endif
