@@ -233,3 +233,84 @@ def setup_sphere(s: I.Solver, nx: int = 256, ny: int = 128, nz: int = 128, Re: f
     s.assemble()
     s.init_flow_const(1.0, 0.0, 0.0, 0.0)
     return s
+
+
+# ---------------------------------------------------------------------------
+# LES channel at Re_tau = 180 (the reference's src/channel.c:47-60 set-up with
+# y as the wall-normal axis): periodic x and z, stationary walls at y = +-1,
+# y_j = -cos(pi j / Ny), constant mean pressure gradient f = 1 (u_tau = 1,
+# delta = 1, nu = 1 / 180).
+# ---------------------------------------------------------------------------
+CHANNEL_SEED = 3
+CHANNEL_BULK = 15.6  # bulk velocity of the initial parabola, in u_tau
+
+
+def channel_faces(nx: int = 192, ny: int = 96, nz: int = 128):
+    return (uniform_faces(0.0, 4 * math.pi, nx),
+            np.array([-math.cos(math.pi * j / ny) for j in range(ny + 1)]),
+            uniform_faces(0.0, 4 * math.pi / 3, nz))
+
+
+def channel_initial(xc, yc, zc, seed: int = CHANNEL_SEED, bulk: float = CHANNEL_BULK, amp: float = 0.1):
+    """Laminar parabola u = 1.5 bulk (1 - y^2) plus seeded Gaussian noise
+    made solenoidal: the noise is the curl of a vector potential whose
+    components are independent Gaussian draws (one per padded cell, seed)
+    damped by (1 - y^2)^2 toward the walls, by central differences (periodic
+    in x and z), scaled to an RMS of amp * bulk over the interior. Arrays are
+    over the padded cell centres (the reference layout)."""
+    rng = np.random.default_rng(seed)
+    X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+    damp = np.clip(1 - Y**2, 0.0, None) ** 2
+    psi = [damp * rng.standard_normal(X.shape) for _ in range(3)]
+
+    def d(f, axis, c):
+        # central difference on the padded array, periodic wrap where it runs
+        # past the ends (the padding layers are rewritten by the solver)
+        fp, fm = np.roll(f, -1, axis), np.roll(f, 1, axis)
+        cc = np.asarray(c)
+        h = np.roll(cc, -1) - np.roll(cc, 1)
+        h[0], h[-1] = cc[1] - cc[0], cc[-1] - cc[-2]
+        shape = [1, 1, 1]
+        shape[axis] = -1
+        return (fp - fm) / h.reshape(shape)
+    u = d(psi[2], 1, yc) - d(psi[1], 2, zc)
+    v = d(psi[0], 2, zc) - d(psi[2], 0, xc)
+    w = d(psi[1], 0, xc) - d(psi[0], 1, yc)
+    inner = (slice(2, -2),) * 3
+    rms = math.sqrt(np.mean(u[inner] ** 2 + v[inner] ** 2 + w[inner] ** 2) / 3)
+    s = amp * bulk / rms
+    return 1.5 * bulk * np.clip(1 - Y**2, 0.0, None) + s * u, s * v, s * w
+
+
+def setup_channel(s: I.Solver, nx: int = 192, ny: int = 96, nz: int = 128, dt: float = 0.002,
+                  tol: float = 1e-9, precision: int = I.PREC_FP64, turb: tuple | None = None,
+                  seed: int = CHANNEL_SEED):
+    """The LES channel workload: turb = (TURBMODEL_SMAGORINSKY_VD, Cs) for
+    Smagorinsky with van Driest damping at the y walls (A+ = 26, u_tau = 1),
+    (TURBMODEL_VREMAN, c) for Vreman, (TURBMODEL_SMAGORINSKY, Cs) for the
+    reference's model, None for no model."""
+    xf, yf, zf = channel_faces(nx, ny, nz)
+    s.set_grid(xf, yf, zf)
+    s.set_params(180.0, dt)
+    s.set_bc(I.DIR_WEST | I.DIR_EAST | I.DIR_DOWN | I.DIR_UP, I.BC_ALL_PERIODIC)
+    s.set_bc(I.DIR_NORTH | I.DIR_SOUTH, I.BC_STATIONARY_WALL)
+    s.set_ext_force(1.0, 0.0, 0.0)
+    if turb is not None:
+        model, coef = turb
+        if model == I.TURBMODEL_SMAGORINSKY_VD:
+            s.set_turb_model(model, coef, aplus=26.0, u_tau=1.0, walls=I.DIR_NORTH | I.DIR_SOUTH)
+        else:
+            s.set_turb_model(model, coef)
+    s.set_obstacle(None)
+    s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, tol)
+    s.set_precision(precision)
+    s.assemble()
+    xc = cell_centres(xf, periodic=True)
+    yc = cell_centres(yf)
+    zc = cell_centres(zf, periodic=True)
+    u, v, w = channel_initial(xc, yc, zc, seed)
+    s.init_flow_const(0, 0, 0, 0)
+    s.set("U1", u)
+    s.set("U2", v)
+    s.set("U3", w)
+    return s

@@ -222,6 +222,9 @@ struct _ibm_solver {
     int mg_pre = 2, mg_post = 2; // smoothing sweeps per level
     int mg_smoother = 1;         // 0 fused red-black sweeps, 1 one launch per colour
     int mg_light_from = 1;       // levels >= this (if > 0) smooth with one sweep before and after
+    int mom_solver = 0;          // IBMSolver_set_momentum_solver: 0 automatic, 1 BiCGSTAB
+    int mom_sweeps = -1;         // FP32 sweeps per refinement round of the momentum solves (0: not used; -1: unset)
+    double mom_rate = 0;         // estimated red-black sweep contraction of the momentum operators
 
     // ---- host mirrors ----
     std::vector<double> lvset_h;     // reference layout (k fastest)
@@ -302,6 +305,7 @@ namespace ibm {
 // convective outflow faces' BC-row norms (3 per face, bc[] order)
 constexpr int kSumSlots = 32;
 constexpr int kConvNormSlot = 12;
+constexpr int kConvSpeedSlot = 31;  // convection speed of the face being advanced
 
 // api.cpp
 long host_ref_index(const IBMSolver *s, int i, int j, int k);

@@ -164,9 +164,14 @@ static bool conv_outflow(IBMSolver *s, int init) {
             const int bci = face_bc(ax, hi);
             if (s->bc[bci].type != BC_CONVECTIVE_OUTFLOW) continue;
             any = true;
-            TIMED("conv_outflow", 0,
-                  launch_conv_outflow(s->stream, s->geo, ax, hi, s->bc[bci].uc, s->dt, init, s->u[0].p, s->u[1].p,
-                                      s->u[2].p, s->conv_g[bci].p, s->sums.p + kConvNormSlot + 3 * bci));
+            if ((size_t)conv_outflow_blocks(s->d, ax) * 3 > s->partial.n)
+                fatal("convective outflow: partial-sum buffer too small");
+            TIMED("conv_outflow", 0, {
+                const long nb = launch_conv_outflow(s->stream, s->geo, ax, hi, s->bc[bci].uc, s->dt, init, s->u[0].p,
+                                                    s->u[1].p, s->u[2].p, s->conv_g[bci].p, s->sums.p + kConvSpeedSlot,
+                                                    s->partial.p);
+                launch_sum_partials(s->stream, s->partial.p, nb, 3, s->sums.p + kConvNormSlot + 3 * bci, 0);
+            });
         }
     return any;
 }
@@ -575,7 +580,8 @@ static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const Ghost
                 // the first colour from a zero start writes every cell;
                 // first_done: the kernel that wrote b already wrote it
                 const bool z = zero_init && sw == 0 && c == 0;
-                snprintf(nm, sizeof nm, "rbgs_L%d", level);
+                if (level >= 0) snprintf(nm, sizeof nm, "rbgs_L%d", level);
+                else snprintf(nm, sizeof nm, "mom_rbgs");
                 // Ghost rows are relaxed after each colour, those of the
                 // colour just updated (the others keep their values); past a
                 // zero start that happens inside the colour launch itself.
@@ -597,7 +603,8 @@ static void smooth(IBMSolver *s, const Dims &d, const SysCoef<T> &S, const Ghost
     T *cur = x, *nxt = xalt;
     for (int sw = 0; sw < sweeps; sw++) {
         const bool z = zero_init && sw == 0;
-        snprintf(nm, sizeof nm, "rbgs_fused_L%d", level);
+        if (level >= 0) snprintf(nm, sizeof nm, "rbgs_fused_L%d", level);
+        else snprintf(nm, sizeof nm, "mom_rbgs_fused");
         TIMED(nm, n * ((z ? 2 : 3) * sizeof(T) + 1),
               launch_rbgs_fused<T>(s->stream, d, S, flag, b, cur, nxt, z ? 1 : 0));
         if (G.ng) {
@@ -1337,6 +1344,108 @@ static void solve_system(IBMSolver *s, int sk, const double *b, double *x, doubl
 }
 
 // ---------------------------------------------------------------------------
+// Momentum solves of the mixed-Krylov mode: FP32 red-black Gauss-Seidel
+// sweeps inside FP64 iterative refinement. The Helmholtz operator
+// (1 + sum K) I - sum K_nb is strongly diagonally dominant wherever
+// dt / (Re h^2) is small: a Jacobi sweep then contracts fluid rows by
+// rho_J = max sum K / (1 + sum K) and a red-black sweep by about rho_J^2, so
+// m sweeps per round reduce the correction's residual by rho_J^(2m) until FP32
+// rounding (~1e-7) limits it. Each round: FP32 right-hand side (the FP64
+// residual; the first colour written in the same launch), m sweeps from zero,
+// x += e in FP64, FP64 residual and its norm. The stopping test is the
+// reference's ||b - A x|| <= tol ||b|| in FP64. Where rho_J^2 > 0.03 (few
+// sweeps would not do) the BiCGSTAB path solves instead, as it does for a
+// component whose residual stops falling by at least 10x per round.
+// ---------------------------------------------------------------------------
+static const double kMomRoundTarget = 1e-6;  // residual reduction aimed at per round
+static const double kMomMaxRate = 0.03;      // largest rho_J^2 for which sweeps are used
+static const int kMomMaxRounds = 8;
+
+static int momentum_sweeps(IBMSolver *s) {
+    if (s->mom_sweeps >= 0) return s->mom_sweeps;
+    double mx[3] = {0, 0, 0};
+    for (int i = 0; i < s->Nx; i++) mx[0] = std::max(mx[0], H1(s->kW, i) + H1(s->kE, i));
+    for (int j = 0; j < s->Ny; j++) mx[1] = std::max(mx[1], H1(s->kS, j) + H1(s->kN, j));
+    for (int k = 0; k < s->Nz; k++) mx[2] = std::max(mx[2], H1(s->kD, k) + H1(s->kU, k));
+    const double sk = mx[0] + mx[1] + mx[2], rj = sk / (1 + sk), rate = rj * rj;
+    s->mom_rate = rate;
+    s->mom_sweeps = (rate > 0 && rate <= kMomMaxRate)
+                        ? std::max(2, (int)std::ceil(std::log(kMomRoundTarget) / std::log(rate))) : 0;
+    return s->mom_sweeps;
+}
+
+// Sweeps for a round that must reduce the residual by `need` (< 1): enough
+// at the estimated rate with a factor 2 to spare, at least one, at most the
+// full round's count.
+static int momentum_round_sweeps(const IBMSolver *s, double need) {
+    const double f = std::max(need, kMomRoundTarget) * 0.5;
+    return std::min(s->mom_sweeps, std::max(1, (int)std::ceil(std::log(f) / std::log(s->mom_rate))));
+}
+
+// The three momentum solves by refinement; components that do not converge
+// this way are solved by solve_system. bn: the stopping norms.
+static void momentum_refine(IBMSolver *s, const double bn[3]) {
+    const Dims &d = s->d;
+    const int m = momentum_sweeps(s);
+    const double cb = ncells(s);
+    const long na = apply_partials(d, s->gmom.G64.ng);
+    const NormSpec off{0, nullptr, nullptr, nullptr, nullptr, -1};
+    double *r64[3] = {s->ref_r64.p, s->mlane[0].v[0].p, s->mlane[1].v[0].p};
+    float *rhs32[3] = {s->ref_rhs32.p, s->mlane[0].d32[0].p, s->mlane[1].d32[0].p};
+    float *e32[3] = {s->ref_e32.p, s->mlane[0].d32[1].p, s->mlane[1].d32[1].p};
+    double rn[3] = {0, 0, 0};
+    int sweeps[3] = {0, 0, 0};
+    bool active[3], fallback[3] = {false, false, false};
+    for (int l = 0; l < 3; l++) {
+        active[l] = bn[l] > 0;
+        if (!active[l]) {
+            launch_zero<double>(s->stream, d, s->us[l].p);
+            report_solve(s, l, 0.0, 0);
+        }
+    }
+    for (int round = 0; round < kMomMaxRounds; round++) {
+        bool any = false;
+        for (int l = 0; l < 3; l++) {
+            if (!active[l]) continue;
+            any = true;
+            const SysCoef<double> &S64 = s->sys[l].S64;
+            const SysCoef<float> &S32 = s->sys[l].S32;
+            // right-hand side of the correction (b itself from x = 0) in FP32,
+            // with the first smoother colour from a zero start
+            // the first round aims at the FP32 limit, later ones at what is left
+            const int mr = round ? momentum_round_sweeps(s, s->tol * bn[l] / rn[l]) : m;
+            TIMED("mom_d2f", cb * 17, launch_d2f_first(s->stream, d, S32, s->flag.p, round ? r64[l] : s->b[l].p,
+                                                       rhs32[l], e32[l]));
+            smooth<float>(s, d, S32, s->gmom.G32, s->flag.p, rhs32[l], e32[l], s->ref_e32b.p, mr, true, -1, true);
+            sweeps[l] += mr;
+            if (round) TIMED("mom_add_f2d", cb * 20, launch_add_f2d(s->stream, d, e32[l], s->us[l].p));
+            else TIMED("mom_f2d", cb * 12, launch_f2d(s->stream, d, e32[l], s->us[l].p));
+            TIMED("mom_residual64", 3 * cb * 8,
+                  launch_residual<double>(s->stream, d, S64, s->gmom.G64, s->flag.p, s->b[l].p, s->us[l].p, r64[l],
+                                          off, s->partial.p));
+            launch_sum_partials(s->stream, s->partial.p, na, 2, s->sums.p + 3 + 2 * l, 0);
+        }
+        if (!any) break;
+        read_sums(s, 9);
+        for (int l = 0; l < 3; l++) {
+            if (!active[l]) continue;
+            const double r = std::sqrt(s->sums_h[3 + 2 * l]);
+            const bool stalled = round > 0 && !(r <= 0.1 * rn[l]);
+            rn[l] = r;
+            if (!std::isfinite(r) || r <= s->tol * bn[l]) {
+                active[l] = false;
+                report_solve(s, l, r / bn[l], sweeps[l]);
+            } else if (stalled) {
+                active[l] = false;
+                fallback[l] = true;
+            }
+        }
+    }
+    for (int l = 0; l < 3; l++)
+        if (active[l] || fallback[l]) solve_system(s, l, s->b[l].p, s->us[l].p, bn[l]);
+}
+
+// ---------------------------------------------------------------------------
 // one fractional step
 // ---------------------------------------------------------------------------
 static StepParams step_params(IBMSolver *s) {
@@ -1429,7 +1538,11 @@ static void do_step(IBMSolver *s) {
     // momentum Helmholtz solves: the three at once on their lanes, or one
     // after the other (mixed defect-correction mode, and under the per-kernel
     // timers, whose events are recorded on the solver's stream)
-    if (s->mom_lanes && s->timing == 0 && s->precision != IBM3D_PREC_MIXED) {
+    if (s->precision == IBM3D_PREC_MIXED_KRYLOV && s->mom_solver == 0 && momentum_sweeps(s) > 0) {
+        double bn[3];
+        mom_norms(bn);
+        momentum_refine(s, bn);
+    } else if (s->mom_lanes && s->timing == 0 && s->precision != IBM3D_PREC_MIXED) {
         momentum_concurrent(s, mom_norms);
     } else {
         double bn[3];
@@ -1584,6 +1697,11 @@ void IBMSolver_enable_timing(IBMSolver *s, int mode) {
 }
 
 void IBMSolver_set_graph_replay(IBMSolver *s, int on) { s->graphs = on != 0; }
+
+void IBMSolver_set_momentum_solver(IBMSolver *s, int mode) {
+    if (mode < 0 || mode > 1) fatal("momentum solver mode must be 0 or 1");
+    s->mom_solver = mode;
+}
 
 void IBMSolver_reset_timing(IBMSolver *s) {
     timers_flush(s);

@@ -54,11 +54,15 @@ struct OuterParams {
 
 // Convective outflow face values of one face (bci = N,E,S,W,D,U index):
 // init != 0 copies the first interior cell's velocity; otherwise one upwind
-// step of dg/dt + Uc dg/dn = 0. garr holds 4 components x the face's
-// tangential extent (the 4th, the pressure value, stays 0); norm_out[0..2]
-// receives 2 sum g_c^2 over the face (the BC rows' share of |b|^2).
-void launch_conv_outflow(hipStream_t st, const Geo &g, int ax, int hi, double uc, double dt, int init,
-                         const double *u1, const double *u2, const double *u3, double *garr, double *norm_out);
+// step of dg/dt + Uc dg/dn = 0, with Uc = uc or (uc <= 0) the face's mean
+// outward velocity, written to *uc_slot. garr holds 4 components x the
+// face's tangential extent (the 4th, the pressure value, stays 0). Leaves
+// per-block partial sums of 2 g_c^2 (the BC rows' share of |b|^2, 3 per
+// block) in partial; returns the block count.
+long conv_outflow_blocks(const Dims &d, int ax);
+long launch_conv_outflow(hipStream_t st, const Geo &g, int ax, int hi, double uc, double dt, int init,
+                         const double *u1, const double *u2, const double *u3, double *garr, double *uc_slot,
+                         double *partial);
 
 void launch_interp_stag(hipStream_t st, const Geo &g, const double *u1, const double *u2, const double *u3,
                         double *U1, double *U2, double *U3);

@@ -385,8 +385,10 @@ __global__ void k_outer(Geo g, OuterParams op, int ax, double *f1, double *f2, d
 }
 
 // ---------------------------------------------------------------------------
-// Convective outflow (extension; no counterpart in the reference). One
-// workgroup per face, so the face sums below have one fixed order.
+// Convective outflow (extension; no counterpart in the reference). The
+// face's convection speed comes from one workgroup (fixed-order sums); the
+// update of the face values runs one thread per face entry over many
+// workgroups, each leaving fixed-order partial sums of its BC-row norms.
 // ---------------------------------------------------------------------------
 static constexpr int kConvThreads = 1024;
 
@@ -403,63 +405,96 @@ __device__ inline double conv_block_sum(double v, double *red) {
     return r;
 }
 
+struct ConvFace {
+    int a0, a1, b0, b1, n2, q0;
+    long ext, st;
+    double bi;
+};
+
+__device__ inline ConvFace conv_face(const Geo &g, int ax, int hi) {
+    const Dims &d = g.d;
+    ConvFace f;
+    if (ax == 0) { f.a0 = 0; f.a1 = d.ny; f.b0 = 0; f.b1 = d.nz; }
+    else if (ax == 1) { f.a0 = -2; f.a1 = d.nx + 2; f.b0 = 0; f.b1 = d.nz; }
+    else { f.a0 = -2; f.a1 = d.nx + 2; f.b0 = -2; f.b1 = d.ny + 2; }
+    f.n2 = f.b1 - f.b0;
+    f.ext = (long)(f.a1 - f.a0) * f.n2;
+    const int N = ax == 0 ? d.nx : ax == 1 ? d.ny : d.nz;
+    f.q0 = hi ? N - 1 : 0;
+    f.st = ax == 0 ? 1 : ax == 1 ? d.sy : d.sz;
+    f.bi = C1(ax == 0 ? g.dx : ax == 1 ? g.dy : g.dz, f.q0) / 2;
+    return f;
+}
+
+// uc_out = the set speed uc, or for uc <= 0 the area-weighted mean outward
+// normal component of the face values over the face's interior cells,
+// clamped at 0
 __global__ void __launch_bounds__(kConvThreads)
-k_conv_outflow(Geo g, int ax, int hi, double uc, double dt, int init, const double *u1, const double *u2,
-               const double *u3, double *garr, double *norm_out) {
+k_conv_speed(Geo g, int ax, int hi, double uc, const double *garr, double *uc_out) {
     __shared__ double red[kConvThreads];
     const Dims &d = g.d;
-    int a0, a1, b0, b1;
-    if (ax == 0) { a0 = 0; a1 = d.ny; b0 = 0; b1 = d.nz; }
-    else if (ax == 1) { a0 = -2; a1 = d.nx + 2; b0 = 0; b1 = d.nz; }
-    else { a0 = -2; a1 = d.nx + 2; b0 = -2; b1 = d.ny + 2; }
-    const int n2 = b1 - b0;
-    const long ext = (long)(a1 - a0) * n2;
-    const int N = ax == 0 ? d.nx : ax == 1 ? d.ny : d.nz;
-    const int q0 = hi ? N - 1 : 0;
-    const long st = ax == 0 ? 1 : ax == 1 ? d.sy : d.sz;
-    const double bi = C1(ax == 0 ? g.dx : ax == 1 ? g.dy : g.dz, q0) / 2;
-    if (!init && uc <= 0) {
-        // area-weighted mean outward normal component over the interior cells
+    if (uc <= 0) {
+        const ConvFace f = conv_face(g, ax, hi);
         const int m1 = ax == 0 ? d.ny : d.nx, m2 = ax == 2 ? d.ny : d.nz;
         const double *w1 = ax == 0 ? g.dy : g.dx, *w2 = ax == 2 ? g.dy : g.dz;
         double num = 0, den = 0;
         for (long e = threadIdx.x; e < (long)m1 * m2; e += kConvThreads) {
             const int t1 = (int)(e / m2), t2 = (int)(e % m2);
             const double A = C1(w1, t1) * C1(w2, t2);
-            num += garr[ax * ext + (long)(t1 - a0) * n2 + (t2 - b0)] * A;
+            num += garr[ax * f.ext + (long)(t1 - f.a0) * f.n2 + (t2 - f.b0)] * A;
             den += A;
         }
         num = conv_block_sum(num, red);
         den = conv_block_sum(den, red);
         uc = fmax((hi ? num : -num) / den, 0.0);
     }
-    const double *u[3] = {u1, u2, u3};
+    if (threadIdx.x == 0) *uc_out = uc;
+}
+
+// init: g = the first interior cell's velocity; otherwise one explicit
+// upwind step g <- g - dt uc (g - u_q0) / (d_q0 / 2). One thread per face
+// entry; partial[blk * 3 + c] = the block's sum of 2 g_c^2.
+__global__ void __launch_bounds__(256)
+k_conv_update(Geo g, int ax, int hi, const double *uc_in, double dt, int init, const double *u1, const double *u2,
+              const double *u3, double *garr, double *partial) {
+    const Dims &d = g.d;
+    const ConvFace f = conv_face(g, ax, hi);
+    const long e = blockIdx.x * 256L + threadIdx.x;
     double nrm[3] = {0, 0, 0};
-    for (long e = threadIdx.x; e < ext; e += kConvThreads) {
-        const int t1 = a0 + (int)(e / n2), t2 = b0 + (int)(e % n2);
+    if (e < f.ext) {
+        const double uc = init ? 0.0 : *uc_in;
+        const int t1 = f.a0 + (int)(e / f.n2), t2 = f.b0 + (int)(e % f.n2);
         long base;
         if (ax == 0) base = cidx(d, 0, t1, t2);
         else if (ax == 1) base = cidx(d, t1, 0, t2);
         else base = cidx(d, t1, t2, 0);
-        const long cq = base + q0 * st;
+        const long cq = base + f.q0 * f.st;
+        const double *u[3] = {u1, u2, u3};
+#pragma unroll
         for (int c = 0; c < 3; c++) {
             const double uq = u[c][cq];
-            double gv = garr[c * ext + e];
-            gv = init ? uq : gv - dt * uc * (gv - uq) / bi;
-            garr[c * ext + e] = gv;
-            nrm[c] += 2 * gv * gv;
+            double gv = garr[c * f.ext + e];
+            gv = init ? uq : gv - dt * uc * (gv - uq) / f.bi;
+            garr[c * f.ext + e] = gv;
+            nrm[c] = 2 * gv * gv;
         }
     }
-    for (int c = 0; c < 3; c++) {
-        const double t = conv_block_sum(nrm[c], red);
-        if (threadIdx.x == 0) norm_out[c] = t;
-    }
+    block_sum_store<3>(nrm, partial, blockIdx.x);
 }
 
-void launch_conv_outflow(hipStream_t st, const Geo &g, int ax, int hi, double uc, double dt, int init,
-                         const double *u1, const double *u2, const double *u3, double *garr, double *norm_out) {
-    hipLaunchKernelGGL(k_conv_outflow, dim3(1), dim3(kConvThreads), 0, st, g, ax, hi, uc, dt, init, u1, u2, u3,
-                       garr, norm_out);
+long conv_outflow_blocks(const Dims &d, int ax) {
+    const long ext = ax == 0 ? (long)d.ny * d.nz : ax == 1 ? (long)(d.nx + 4) * d.nz : (long)(d.nx + 4) * (d.ny + 4);
+    return (ext + 255) / 256;
+}
+
+long launch_conv_outflow(hipStream_t st, const Geo &g, int ax, int hi, double uc, double dt, int init,
+                         const double *u1, const double *u2, const double *u3, double *garr, double *uc_slot,
+                         double *partial) {
+    if (!init) hipLaunchKernelGGL(k_conv_speed, dim3(1), dim3(kConvThreads), 0, st, g, ax, hi, uc, garr, uc_slot);
+    const long nblk = conv_outflow_blocks(g.d, ax);
+    hipLaunchKernelGGL(k_conv_update, dim3(nblk), dim3(256), 0, st, g, ax, hi, uc_slot, dt, init, u1, u2, u3, garr,
+                       partial);
+    return nblk;
 }
 
 // ---------------------------------------------------------------------------

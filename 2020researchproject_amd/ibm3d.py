@@ -83,6 +83,7 @@ _SIGS = {
 _OPTIONAL = {
     "IBMSolver_calc_body_force": (None, [_vp, _dp]),
     "IBMSolver_set_force_history": (None, [_vp, C.c_int]),
+    "IBMSolver_set_momentum_solver": (None, [_vp, C.c_int]),
     "IBMSolver_get_force_history": (C.c_int, [_vp, _dp, C.c_int]),
     "IBMSolver_enable_timing": (None, [_vp, C.c_int]),
     "IBMSolver_set_graph_replay": (None, [_vp, C.c_int]),
@@ -313,6 +314,9 @@ class Solver:
         return out
 
     # --- timing (product only) ----------------------------------------------
+    def set_momentum_solver(self, mode: int):
+        self.lib.IBMSolver_set_momentum_solver(self.h, int(mode))
+
     def set_force_history(self, max_steps: int):
         self.lib.IBMSolver_set_force_history(self.h, int(max_steps))
 

@@ -64,6 +64,15 @@ void IBMSolver_set_mg_params(IBMSolver *, double gamma, int pre, int post, int s
    count on the sphere and cavity cases). Before IBMSolver_assemble. */
 void IBMSolver_set_mg_light_levels(IBMSolver *, int from_level);
 
+/* Momentum Helmholtz solver of the IBM3D_PREC_MIXED_KRYLOV mode. 0
+   (default): FP32 red-black Gauss-Seidel sweeps inside FP64 iterative
+   refinement where the operator is diagonally dominant enough for a few
+   sweeps per round (max over cells of sum K / (1 + sum K) squared <= 0.03),
+   BiCGSTAB elsewhere and for any component whose residual stops falling;
+   1: always BiCGSTAB (FP64, Jacobi preconditioner, FP32 stored directions).
+   Both stop on the reference's test ||b - A x|| <= tol ||b|| in FP64. */
+void IBMSolver_set_momentum_solver(IBMSolver *, int mode);
+
 /* Copies a field to host memory in the reference layout (see above). Solid
    cells hold NaN where the reference stores NaN. */
 void IBMSolver_get_field(IBMSolver *, int field, double *out);

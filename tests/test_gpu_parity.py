@@ -565,3 +565,61 @@ def test_body_force_parity_and_history(new_gpu, new_oracle, cases, I, precision)
     first = h.force_history()
     assert first.shape == (2, 3)
     np.testing.assert_array_equal(first, hist[:2])
+
+
+@pytest.mark.parametrize("case", ["sphere", "cavity"])
+def test_momentum_sweeps_match_bicgstab(new_gpu, cases, I, case):
+    """Mixed-Krylov mode: the momentum solves by FP32 red-black sweeps inside
+    FP64 refinement (the default where the operator is dominant enough) and
+    by BiCGSTAB reach the same fields to the solver tolerance."""
+    res = []
+    for mode in (0, 1):
+        s = new_gpu()
+        if case == "sphere":
+            cases.setup_sphere(s, 64, 32, 32, tol=1e-12, precision=I.PREC_MIXED_KRYLOV)
+        else:
+            cases.setup_cavity(s, n=32, tol=1e-12, precision=I.PREC_MIXED_KRYLOV)
+        s.set_momentum_solver(mode)
+        s.iterate(4)
+        print(f"momentum solver {mode}: its {s.last_iters().tolist()} norms {s.last_norms().tolist()}", flush=True)
+        assert (s.last_norms()[:3] <= 1e-12).all()
+        res.append(s)
+    compare(res[0], res[1], 1e-9)
+
+
+CHANNEL_MODELS = pytest.mark.parametrize("model,coef", [(4, 0.1), (3, 0.07)], ids=["smagorinsky-vd", "vreman"])
+
+
+@CHANNEL_MODELS
+def test_channel_les_mixed_precision_vs_oracle(new_gpu, new_oracle, cases, I, model, coef):
+    """The LES channel workload (Re_tau 180, periodic x and z, cos-clustered
+    walls, seeded solenoidal noise on the parabola) at a reduced grid: the
+    mixed path (FP64 BiCGSTAB, FP32 multigrid) against the FP64 oracle after
+    4 steps, within the mixed path's stated 1e-6 at solver tolerance 1e-10,
+    eddy-viscosity stresses included."""
+    g, o = new_gpu(), new_oracle()
+    cases.setup_channel(g, 32, 24, 16, tol=1e-10, precision=I.PREC_MIXED_KRYLOV, turb=(model, coef))
+    cases.setup_channel(o, 32, 24, 16, tol=1e-10, turb=(model, coef))
+    for s in (g, o):
+        s.iterate(4)
+    compare(g, o, 1e-6)
+    for n in ("TAU11", "TAU12", "TAU23"):
+        assert rel_l2(g.get(n)[INNER], o.get(n)[INNER]) < 1e-6
+    assert g.max_div() < 1e-8
+
+
+@CHANNEL_MODELS
+def test_channel_les_full_size_mixed(new_gpu, cases, I, model, coef):
+    """The channel at its stated 192 x 96 x 128 in mixed precision for a few
+    steps: finite fields, max|div U| <= 1e-8 at tolerance 1e-9, iteration
+    counts and the stopping norms logged."""
+    g = new_gpu()
+    cases.setup_channel(g, tol=1e-9, precision=I.PREC_MIXED_KRYLOV, turb=(model, coef))
+    g.iterate(5)
+    u = g.get("U1")[INNER]
+    print(f"channel 192x96x128 model {model}: its {g.last_iters().tolist()} norms {g.last_norms().tolist()} "
+          f"max|div U| {g.max_div():.2e} mean u {np.nanmean(u):.3f} step ms {np.round(g.step_times(), 2).tolist()}",
+          flush=True)
+    assert np.isfinite(u).all() and np.isfinite(g.get("P")[INNER]).all()
+    assert g.max_div() <= 1e-8
+    assert (g.last_norms() <= 1e-9).all()
