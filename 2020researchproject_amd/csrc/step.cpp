@@ -1679,7 +1679,7 @@ void body_force_read(IBMSolver *s, double out[3]) {
 
 double max_div_impl(IBMSolver *s) {
     const double *U3[3] = {s->F[0].p, s->F[1].p, s->F[2].p};
-    launch_maxdiv(s->stream, s->geo, s->flag.p, U3, s->partial.p);
+    launch_maxdiv(s->stream, s->geo, s->flag.p, s->pin, U3, s->partial.p);
     launch_sum_partials(s->stream, s->partial.p, maxdiv_blocks(s->d), 1, s->sums.p, 1);
     read_sums(s, 1);
     return s->sums_h[0];

@@ -94,6 +94,7 @@ long maxdiv_blocks(const Dims &d);
 long body_force_blocks(int ng);
 void launch_body_force(hipStream_t st, const Geo &g, const uint8_t *flag, const int *gijk, int ng, double mu,
                        const double *p, const double *const u[3], double *partial);
-void launch_maxdiv(hipStream_t st, const Geo &g, const uint8_t *flag, const double *const U[3], double *partial);
+void launch_maxdiv(hipStream_t st, const Geo &g, const uint8_t *flag, long pin, const double *const U[3],
+                   double *partial);
 
 }  // namespace ibm

@@ -660,9 +660,10 @@ __global__ void __launch_bounds__(256) k_sgs(Geo g, int model, double coef, VanD
     for (int a = 0; a < 6; a++) tau.t[a][q] = -2 * nut * S[a];
 }
 
-// max |div U| over fluid cells (partial maxima per block)
-__global__ void __launch_bounds__(256) k_maxdiv(Geo g, const uint8_t *flag, const double *U1, const double *U2,
-                                                const double *U3, double *partial) {
+// max |div U| over fluid cells (partial maxima per block), the pinned cell
+// (whose pressure row is the pin, not continuity) left out
+__global__ void __launch_bounds__(256) k_maxdiv(Geo g, const uint8_t *flag, long pin, const double *U1,
+                                                const double *U2, const double *U3, double *partial) {
     const Dims &d = g.d;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -670,7 +671,7 @@ __global__ void __launch_bounds__(256) k_maxdiv(Geo g, const uint8_t *flag, cons
     double m = 0;
     if (i < d.nx && j < d.ny) {
         const long q = cidx(d, i, j, k);
-        if (flag[q] == F_FLUID) {
+        if (flag[q] == F_FLUID && q != pin) {
             const double v = (U1[q + 1] - U1[q]) / C1(g.dx, i) + (U2[q + d.sy] - U2[q]) / C1(g.dy, j)
                 + (U3[q + d.sz] - U3[q]) / C1(g.dz, k);
             m = fabs(v);
@@ -835,8 +836,9 @@ void launch_body_force(hipStream_t st, const Geo &g, const uint8_t *flag, const 
                        u[1], u[2], partial);
 }
 
-void launch_maxdiv(hipStream_t st, const Geo &g, const uint8_t *flag, const double *const U[3], double *partial) {
-    hipLaunchKernelGGL(k_maxdiv, cell_grid(g.d, 0), dim3(64, 4), 0, st, g, flag, U[0], U[1], U[2], partial);
+void launch_maxdiv(hipStream_t st, const Geo &g, const uint8_t *flag, long pin, const double *const U[3],
+                   double *partial) {
+    hipLaunchKernelGGL(k_maxdiv, cell_grid(g.d, 0), dim3(64, 4), 0, st, g, flag, pin, U[0], U[1], U[2], partial);
 }
 
 }  // namespace ibm

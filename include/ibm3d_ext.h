@@ -98,7 +98,10 @@ int IBMSolver_get_ghost_cells(IBMSolver *, int *out);
    when out != NULL, returns the number of levels. */
 int IBMSolver_get_mg_levels(IBMSolver *, int *out, int max);
 
-/* max |div U| over fluid cells, U the face velocity after projection. */
+/* max |div U| over fluid cells, U the face velocity after projection. Where
+   no pressure-type boundary exists, the pressure row of cell (0,0,0) is the
+   pin p' = 0 (src/ibm3d_setup.c:1598-1624), not continuity; that cell takes
+   up the residual of all other rows and is left out. */
 double IBMSolver_max_div(IBMSolver *);
 
 /* Force on the immersed body per unit density (Fx, Fy, Fz), pressure plus

@@ -1643,12 +1643,18 @@ int IBMSolver_get_ghost_cells(IBMSolver *s, int *out) {
     return n;
 }
 
+/* max |div U| over the fluid cells whose row of the pressure system is the
+   continuity equation: where no pressure-type boundary exists, the row of
+   cell (0,0,0) pins p' instead (src/ibm3d_setup.c:1598-1624) and that cell
+   takes up the residual of every other row, so it is left out. */
 double IBMSolver_max_div(IBMSolver *s) {
     double m = 0;
+    const bool sing = p_singular(s);
     for (int i = 0; i < s->Nx; i++)
         for (int j = 0; j < s->Ny; j++)
             for (int k = 0; k < s->Nz; k++) {
                 if (C3(s, s->flag, i, j, k) != FLAG_FLUID) continue;
+                if (sing && i == 0 && j == 0 && k == 0) continue;
                 const double d = (XS(s, s->U1, i + 1, j, k) - XS(s, s->U1, i, j, k)) / C1(s->dx, i)
                     + (YS(s, s->U2, i, j + 1, k) - YS(s, s->U2, i, j, k)) / C1(s->dy, j)
                     + (ZS(s, s->U3, i, j, k + 1) - ZS(s, s->U3, i, j, k)) / C1(s->dz, k);
