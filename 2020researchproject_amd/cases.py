@@ -314,3 +314,100 @@ def setup_channel(s: I.Solver, nx: int = 192, ny: int = 96, nz: int = 128, dt: f
     s.set("U2", v)
     s.set("U3", w)
     return s
+
+
+# ---------------------------------------------------------------------------
+# Urban canopy: 16 axis-aligned blocks on the ground of a 16 x 8 x 4 domain
+# (x streamwise, y spanwise, z up), triangulated to a binary STL in memory and
+# read through Polyhedron_read_stl as the reference's building driver reads
+# its STL (src/building.c:61-75), log-law inflow by VAL_FUNC, Vreman LES.
+# ---------------------------------------------------------------------------
+URBAN_SEED = 4
+URBAN_DOMAIN = (16.0, 8.0, 4.0)
+URBAN_USTAR, URBAN_Z0, KARMAN = 0.05, 0.01, 0.41
+URBAN_BASE = -0.5  # blocks reach below the ground so no ghost stencil crosses it
+
+
+def urban_blocks(seed: int = URBAN_SEED, n: int = 16, gap: float = 0.3, x_range=(1.5, 12.5)):
+    """n blocks (x0, x1, y0, y1, height): footprints U[0.5, 1.5]^2, heights
+    U[0.5, 2.0], centres uniform over x_range x [0, 8] with the footprint
+    inside the domain, at least `gap` apart (rejection sampling, seeded)."""
+    rng = np.random.default_rng(seed)
+    Ly = URBAN_DOMAIN[1]
+    out = []
+    while len(out) < n:
+        wx, wy = rng.uniform(0.5, 1.5, 2)
+        h = rng.uniform(0.5, 2.0)
+        cx = rng.uniform(x_range[0] + wx / 2, x_range[1] - wx / 2)
+        cy = rng.uniform(wy / 2 + gap, Ly - wy / 2 - gap)
+        b = (cx - wx / 2, cx + wx / 2, cy - wy / 2, cy + wy / 2, h)
+        if all(b[0] >= o[1] + gap or o[0] >= b[1] + gap or b[2] >= o[3] + gap or o[2] >= b[3] + gap for o in out):
+            out.append(b)
+    return out
+
+
+def blocks_stl(blocks, base: float = URBAN_BASE) -> bytes:
+    """Binary STL of closed boxes: 12 triangles each, counter-clockwise seen
+    from outside, normals outward."""
+    tris = []
+    for (x0, x1, y0, y1, h) in blocks:
+        z0, z1 = base, h
+        v = [(x, y, z) for z in (z0, z1) for y in (y0, y1) for x in (x0, x1)]  # index x + 2y + 4z
+        quads = [((0, 2, 3, 1), (0, 0, -1)), ((4, 5, 7, 6), (0, 0, 1)), ((0, 1, 5, 4), (0, -1, 0)),
+                 ((2, 6, 7, 3), (0, 1, 0)), ((0, 4, 6, 2), (-1, 0, 0)), ((1, 3, 7, 5), (1, 0, 0))]
+        for (a, b, c, d), nrm in quads:
+            tris.append((nrm, v[a], v[b], v[c]))
+            tris.append((nrm, v[a], v[c], v[d]))
+    out = bytearray(b"urban canopy blocks".ljust(80, b" "))
+    out += struct.pack("<I", len(tris))
+    for nrm, a, b, c in tris:
+        out += struct.pack("<12fH", *nrm, *a, *b, *c, 0)
+    return bytes(out)
+
+
+def urban_faces(nx: int = 320, ny: int = 160, nz: int = 96, dmin: float = 0.01):
+    """Uniform in x and y; z clustered toward the ground by a partial cosine
+    with first cell dmin (at the 96-cell grid; scaled by 96 / nz on others)."""
+    Lx, Ly, Lz = URBAN_DOMAIN
+    d0 = dmin * 96.0 / nz
+    lo, hi = 1e-6, math.pi / 2 - 1e-6
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if _partial_cos(Lz, nz, mid)[1] < d0:
+            lo = mid
+        else:
+            hi = mid
+    return uniform_faces(0.0, Lx, nx), uniform_faces(0.0, Ly, ny), _partial_cos(Lz, nz, 0.5 * (lo + hi))
+
+
+def log_law(z: float) -> float:
+    """Inflow u(z) = (u* / kappa) ln((z + z0) / z0)."""
+    return URBAN_USTAR / KARMAN * math.log((max(z, 0.0) + URBAN_Z0) / URBAN_Z0)
+
+
+def setup_urban(s: I.Solver, nx: int = 320, ny: int = 160, nz: int = 96, Re: float = 2.0e4, dt: float = 0.001,
+                tol: float = 1e-9, precision: int = I.PREC_FP64, turb: tuple | None = (I.TURBMODEL_VREMAN, 0.07),
+                seed: int = URBAN_SEED):
+    """The urban-canopy workload: West log-law inflow (VAL_FUNC), East
+    pressure outlet, free-slip lateral and top boundaries, no-slip ground;
+    blocks from urban_blocks(seed) through an in-memory STL; initial field the
+    inflow profile everywhere."""
+    xf, yf, zf = urban_faces(nx, ny, nz)
+    s.set_grid(xf, yf, zf)
+    s.set_params(Re, dt)
+    zero = lambda t, x, y, z: 0.0
+    s.set_bc(I.DIR_WEST, I.BC_VELOCITY_COMPONENT, I.VAL_FUNC, lambda t, x, y, z: log_law(z), zero, zero)
+    s.set_bc(I.DIR_EAST, I.BC_PRESSURE, I.VAL_CONST, 0.0)
+    s.set_bc(I.DIR_NORTH | I.DIR_SOUTH | I.DIR_UP, I.BC_FREE_SLIP_WALL)
+    s.set_bc(I.DIR_DOWN, I.BC_STATIONARY_WALL)
+    if turb is not None:
+        s.set_turb_model(*turb)
+    s.set_obstacle_stl_bytes(blocks_stl(urban_blocks(seed)))
+    s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, tol)
+    s.set_precision(precision)
+    s.assemble()
+    zc = cell_centres(zf)
+    u = np.array([log_law(z) for z in zc])
+    s.init_flow_const(0, 0, 0, 0)
+    s.set("U1", np.broadcast_to(u[None, None, :], s.shape).copy())
+    return s

@@ -222,6 +222,25 @@ class Solver:
         self._poly = poly
         self.lib.IBMSolver_set_obstacle(self.h, poly)
 
+    def set_obstacle_stl_bytes(self, data: bytes):
+        """Reads a binary STL held in memory through Polyhedron_read_stl (an
+        in-memory FILE* from fmemopen, so the reference's reader API is used
+        unchanged) and sets it as the obstacle, borrowed until assemble."""
+        import ctypes.util
+        libc = C.CDLL(ctypes.util.find_library("c"))
+        libc.fmemopen.restype = C.c_void_p
+        libc.fmemopen.argtypes = [C.c_void_p, C.c_size_t, C.c_char_p]
+        libc.fclose.argtypes = [C.c_void_p]
+        buf = C.create_string_buffer(bytes(data), len(data))
+        fp = libc.fmemopen(buf, len(data), b"rb")
+        if not fp:
+            raise OSError("fmemopen failed")
+        poly = self.lib.Polyhedron_new()
+        self.lib.Polyhedron_read_stl(poly, fp)
+        libc.fclose(fp)
+        self._poly = poly
+        self.lib.IBMSolver_set_obstacle(self.h, poly)
+
     def set_linear_solver(self, solver=SOLVER_BiCGSTAB, precond=PRECOND_AMG, tol=1e-10):
         self.lib.IBMSolver_set_linear_solver(self.h, solver, precond, tol)
 

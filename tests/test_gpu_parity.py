@@ -626,3 +626,46 @@ def test_channel_les_full_size_mixed(new_gpu, cases, I, model, coef):
           flush=True)
     assert np.isfinite(u).all() and np.isfinite(g.get("P")[INNER]).all()
     assert g.max_div() <= 1e-8
+
+
+def test_urban_canopy_flags_and_level_set(new_gpu, new_oracle, cases, I):
+    """Urban canopy (16 seeded blocks triangulated to an in-memory STL, read
+    through Polyhedron_read_stl): the GPU level set within 1e-12 of the
+    oracle's CPT, flags and ghost-cell sets bit-exact, at 80 x 40 x 24."""
+    g, o = new_gpu(), new_oracle()
+    for s in (g, o):
+        cases.setup_urban(s, 80, 40, 24, tol=1e-10)
+    assert np.array_equal(g.flags(), o.flags())
+    assert np.array_equal(g.ghost_cells(), o.ghost_cells()) and len(g.ghost_cells()) > 1000
+    np.testing.assert_allclose(g.get("LVSET"), o.get("LVSET"), rtol=0, atol=1e-12)
+
+
+def test_urban_canopy_mixed_precision_vs_oracle(new_gpu, new_oracle, cases, I):
+    """The urban canopy in mixed precision (log-law VAL_FUNC inflow, Vreman)
+    against the FP64 oracle after 3 steps at 80 x 40 x 24, within 1e-6."""
+    g, o = new_gpu(), new_oracle()
+    cases.setup_urban(g, 80, 40, 24, tol=1e-10, precision=I.PREC_MIXED_KRYLOV)
+    cases.setup_urban(o, 80, 40, 24, tol=1e-10)
+    for s in (g, o):
+        s.iterate(3)
+    compare(g, o, 1e-6)
+    for n in ("TAU11", "TAU13"):
+        assert rel_l2(g.get(n)[INNER], o.get(n)[INNER]) < 1e-6
+    assert g.max_div() < 1e-8
+
+
+def test_urban_canopy_full_size_mixed(new_gpu, cases, I):
+    """The urban canopy at its stated 320 x 160 x 96 in mixed precision for a
+    few steps (level set from the 192 STL triangles on the GPU): finite
+    fields, max|div U| <= 1e-8, iteration counts and timings logged."""
+    import time
+    g = new_gpu()
+    t0 = time.perf_counter()
+    cases.setup_urban(g, tol=1e-10, precision=I.PREC_MIXED_KRYLOV)
+    t_setup = time.perf_counter() - t0
+    g.iterate(3)
+    print(f"urban 320x160x96: setup {t_setup:.2f} s, ghost cells {len(g.ghost_cells())}, its "
+          f"{g.last_iters().tolist()} norms {g.last_norms().tolist()} max|div U| {g.max_div():.2e} step ms "
+          f"{np.round(g.step_times(), 2).tolist()}", flush=True)
+    assert np.isfinite(g.get("U1")[INNER][g.flags()[INNER] == I.FLAG_FLUID]).all()
+    assert g.max_div() <= 1e-8
