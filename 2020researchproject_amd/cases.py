@@ -137,8 +137,12 @@ CAVITY_SEED = 2020
 def setup_cavity(s: I.Solver, n: int = 32, Re: float = 100.0, dt: float = 0.005,
                  tol: float = 1e-10, seed: int = CAVITY_SEED, amp: float = 1e-3,
                  precision: int = I.PREC_FP64, init: str = "array", shape=None,
-                 solver: int = I.SOLVER_BiCGSTAB, precond: int = I.PRECOND_AMG):
-    """Lid-driven cavity on the unit cube, n^3 cells (or shape = (nx, ny, nz))."""
+                 solver: int = I.SOLVER_BiCGSTAB, precond: int = I.PRECOND_AMG, assemble: bool = True):
+    """Lid-driven cavity on the unit cube, n^3 cells (or shape = (nx, ny, nz)).
+    assemble=False stops before IBMSolver_assemble and returns the initial
+    flow as an IBMServeInit (rest plus the seeded perturbation of the
+    interior cells, the cells whose centres lie inside the cube), for a
+    serving job that assembles and initialises natively."""
     nx, ny, nz = shape if shape is not None else (n, n, n)
     fx, fy, fz = uniform_faces(0.0, 1.0, nx), uniform_faces(0.0, 1.0, ny), uniform_faces(0.0, 1.0, nz)
     s.set_grid(fx, fy, fz)
@@ -148,6 +152,8 @@ def setup_cavity(s: I.Solver, n: int = 32, Re: float = 100.0, dt: float = 0.005,
     s.set_obstacle(None)
     s.set_linear_solver(solver, precond, tol)
     s.set_precision(precision)
+    if not assemble:
+        return I.ServeInit(0.0, 0.0, 0.0, 0.0, amp, seed)
     s.assemble()
 
     def pert(comp):
@@ -202,13 +208,15 @@ def sphere_faces(nx: int = 256, ny: int = 128, nz: int = 128, dmin: float = SPHE
 def setup_sphere(s: I.Solver, nx: int = 256, ny: int = 128, nz: int = 128, Re: float = 300.0,
                  dt: float = 0.002, tol: float = 1e-10, precision: int = I.PREC_FP64,
                  dmin: float | None = None, faces=None, solver: int = I.SOLVER_BiCGSTAB,
-                 precond: int = I.PRECOND_AMG, turb: tuple | None = None, outflow: str = "pressure"):
+                 precond: int = I.PRECOND_AMG, turb: tuple | None = None, outflow: str = "pressure",
+                 assemble: bool = True):
     """Flow past the analytic sphere phi = |x| - 0.5: velocity inlet (1, 0, 0)
     on the West face, free-slip lateral walls, and on the East face a
     pressure outlet p = 0 (outflow "pressure", the reference's BC_PRESSURE) or
     a convective outflow advected with the mean outlet velocity ("convective",
     BC_CONVECTIVE_OUTFLOW); turb = (model, coefficient) switches on an
-    eddy-viscosity model."""
+    eddy-viscosity model. assemble=False stops before IBMSolver_assemble and
+    returns the initial flow (uniform u = 1) as an IBMServeInit."""
     if faces is None:
         if dmin is None:
             # keep the cell-width ratio of the 256x128x128 grid on coarser grids
@@ -230,6 +238,8 @@ def setup_sphere(s: I.Solver, nx: int = 256, ny: int = 128, nz: int = 128, Re: f
         s.set_turb_model(*turb)
     s.set_linear_solver(solver, precond, tol)
     s.set_precision(precision)
+    if not assemble:
+        return I.ServeInit(1.0, 0.0, 0.0, 0.0, 0.0, 0)
     s.assemble()
     s.init_flow_const(1.0, 0.0, 0.0, 0.0)
     return s

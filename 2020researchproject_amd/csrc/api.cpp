@@ -18,8 +18,24 @@ using namespace ibm;
 
 namespace ibm {
 
-void fatal(const char *msg) {
-    fprintf(stderr, "error: %s\n", msg);
+void case_error(const std::string &msg, bool on_stdout) {
+    FILE *f = on_stdout ? stdout : stderr;
+    fputs(msg.c_str(), f);
+    if (msg.empty() || msg.back() != '\n') fputc('\n', f);
+    fflush(stdout);
+    fflush(stderr);
+    if (tl_job_errors_throw) throw JobError(msg);
+    abort();
+}
+
+void fatal(const char *msg) { case_error(std::string("error: ") + msg); }
+
+void hip_failure(hipError_t e, const char *expr, const char *file, int line) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "error: HIP call %s failed at %s:%d: %s", expr, file, line, hipGetErrorString(e));
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) case_error(buf);
+    fprintf(stderr, "%s\n", buf);
+    fflush(stdout);
     fflush(stderr);
     abort();
 }
@@ -44,10 +60,10 @@ extern "C" {
 
 IBMSolver *IBMSolver_new(const int num_process, const int rank, const int Px, const int Py, const int Pz) {
     if (num_process != 1 || rank != 0 || Px * Py * Pz != 1) {
-        fprintf(stderr, "error: one GPU holds the whole domain: num_process must be 1 (got %d, %dx%dx%d)\n",
-                num_process, Px, Py, Pz);
-        fflush(stdout);
-        abort();
+        char buf[160];
+        snprintf(buf, sizeof buf, "error: one GPU holds the whole domain: num_process must be 1 (got %d, %dx%dx%d)",
+                 num_process, Px, Py, Pz);
+        case_error(buf);
     }
     // the device resources are created by IBMSolver_assemble once its input
     // checks pass (init_device), so a rejected setup never touches the GPU
@@ -60,7 +76,10 @@ void IBMSolver_destroy(IBMSolver *s) {
         delete s;
         return;
     }
+    // a job that failed part-way may have left work queued on any stream
     (void)hipStreamSynchronize(s->stream);
+    for (auto &L : s->mlane)
+        if (L.st) (void)hipStreamSynchronize(L.st);
     HipGlobalLock lk(hip_global_mutex());
     for (auto &t : s->timers)
         for (auto &e : t.pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
@@ -646,26 +665,18 @@ extern "C" void IBMSolver_assemble(IBMSolver *s) {
     // <- src/ibm3d_setup.c:523-533
     if (isperiodic(s->bc[0].type) != isperiodic(s->bc[2].type) || isperiodic(s->bc[1].type) != isperiodic(s->bc[3].type)
         || isperiodic(s->bc[4].type) != isperiodic(s->bc[5].type)) {
-        printf("Inconsistent periodic boundary condition\n");
-        fflush(stdout);
-        abort();
+        case_error("Inconsistent periodic boundary condition\n", true);
     }
     // pressure solver and preconditioner types  <- src/ibm3d_setup.c:1435-1536
     if (s->solver_type != SOLVER_AMG && s->solver_type != SOLVER_PCG && s->solver_type != SOLVER_BiCGSTAB &&
         s->solver_type != SOLVER_GMRES) {
-        printf("\nUnknown linear solver type\n");
-        fflush(stdout);
-        abort();
+        case_error("\nUnknown linear solver type\n", true);
     }
     if (s->precond_type != PRECOND_NONE && s->precond_type != PRECOND_AMG) {
-        printf("\nUnknown preconditioner type\n");
-        fflush(stdout);
-        abort();
+        case_error("\nUnknown preconditioner type\n", true);
     }
     if (s->solver_type == SOLVER_AMG && s->precond_type != PRECOND_NONE) {
-        printf("\nCannot use preconditioner with BoomerAMG solver\n");
-        fflush(stdout);
-        abort();
+        case_error("\nCannot use preconditioner with BoomerAMG solver\n", true);
     }
     // IBM3D_PREC_MIXED runs BiCGSTAB defect correction for every system
     if (s->precision == IBM3D_PREC_MIXED && s->solver_type != SOLVER_BiCGSTAB)
@@ -679,10 +690,10 @@ extern "C" void IBMSolver_assemble(IBMSolver *s) {
         const char *ax = "xyz";
         for (int a = 0; a < 3; a++)
             if (isperiodic(s->bc[face_bc(a, 0)].type) && n[a] % 2) {
-                fprintf(stderr, "error: periodic %c axis with an odd cell count (%d): the multigrid smoother's "
-                        "red-black colouring needs an even count (or use PRECOND_NONE)\n", ax[a], n[a]);
-                fflush(stdout);
-                abort();
+                char buf[200];
+                snprintf(buf, sizeof buf, "error: periodic %c axis with an odd cell count (%d): the multigrid "
+                         "smoother's red-black colouring needs an even count (or use PRECOND_NONE)", ax[a], n[a]);
+                case_error(buf);
             }
     }
     DevBytesScope account(&s->dev_bytes);

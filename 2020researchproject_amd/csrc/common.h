@@ -13,17 +13,18 @@
 #include <cstdio>
 #include <cstdlib>
 
-#define HIP_CHECK(expr)                                                                   \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess) {                                                           \
-            fprintf(stderr, "error: HIP call %s failed at %s:%d: %s\n", #expr, __FILE__, \
-                    __LINE__, hipGetErrorString(e_));                                     \
-            abort();                                                                      \
-        }                                                                                 \
+#define HIP_CHECK(expr)                                          \
+    do {                                                         \
+        hipError_t e_ = (expr);                                  \
+        if (e_ != hipSuccess) ibm::hip_failure(e_, #expr, __FILE__, __LINE__); \
     } while (0)
 
 namespace ibm {
+
+// Prints the failed call. A failed allocation inside a serving job fails that
+// job (it throws); every other HIP error aborts the process, since the device
+// state of all cases is then in doubt.
+[[noreturn]] void hip_failure(hipError_t e, const char *expr, const char *file, int line);
 
 enum : uint8_t { F_FLUID = 1, F_GHOST = 2, F_SOLID = 3 };
 

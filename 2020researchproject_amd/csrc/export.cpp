@@ -372,8 +372,7 @@ void init_flow_file_impl(IBMSolver *s, const char *filename) {
     NcFile nc;
     FILE *f = fopen(with_ext(filename).c_str(), "rb");
     if (!f) {
-        fprintf(stderr, "error: cannot open file %s\n", filename);
-        abort();
+        case_error(std::string("error: cannot open file ") + filename);
     }
     fseek(f, 0, SEEK_END);
     const long len = ftell(f);
@@ -385,8 +384,7 @@ void init_flow_file_impl(IBMSolver *s, const char *filename) {
     const char *need[6] = {"time", "iter", "u", "v", "w", "p"};
     for (const char *nm : need)
         if (!nc.find(nm)) {
-            fprintf(stderr, "error: varible '%s' not found\n", nm);
-            abort();
+            case_error(std::string("error: varible '") + nm + "' not found");
         }
     const int X = s->Nx + 4, Y = s->Ny + 4, Z = s->Nz + 4;
     const NcFile::V &vt = *nc.find("time"), &vi = *nc.find("iter");
@@ -396,9 +394,10 @@ void init_flow_file_impl(IBMSolver *s, const char *filename) {
     for (const char *nm : fields) {
         const NcFile::V &v = *nc.find(nm);
         if (v.type != NC_FLOAT_T || v.dims != shape) {
-            fprintf(stderr, "error: restart variable '%s' must be float [1][%d][%d][%d] (the solver's padded grid)\n",
-                    nm, Z, Y, X);
-            abort();
+            char buf[200];
+            snprintf(buf, sizeof buf, "error: restart variable '%s' must be float [1][%d][%d][%d] (the solver's "
+                     "padded grid)", nm, Z, Y, X);
+            case_error(buf);
         }
     }
     // the file is checked in full before the solver is touched

@@ -1,6 +1,7 @@
 // solver.h - host-side state of one IBMSolver instance (opaque to callers).
 #pragma once
 
+#include <stdexcept>
 #include <string>
 #include <mutex>
 #include <vector>
@@ -307,9 +308,27 @@ constexpr int kSumSlots = 32;
 constexpr int kConvNormSlot = 12;
 constexpr int kConvSpeedSlot = 31;  // convection speed of the face being advanced
 
+// Errors of one case (rejected input, a non-finite residual, a lost Krylov
+// publication, a failed device allocation). Outside a serving job they print
+// and abort, as the reference does; on a serving worker thread, while the
+// worker runs a job (JobErrorScope), they print and throw JobError, which the
+// worker catches: the job fails, the other cases and the process go on.
+struct JobError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+inline thread_local bool tl_job_errors_throw = false;
+struct JobErrorScope {
+    bool prev;
+    JobErrorScope() : prev(tl_job_errors_throw) { tl_job_errors_throw = true; }
+    ~JobErrorScope() { tl_job_errors_throw = prev; }
+};
+// prints msg (to stdout when on_stdout, else stderr), then throws or aborts
+[[noreturn]] void case_error(const std::string &msg, bool on_stdout = false);
+
 // api.cpp
 long host_ref_index(const IBMSolver *s, int i, int j, int k);
-void fatal(const char *msg);
+// case_error("error: " + msg)
+[[noreturn]] void fatal(const char *msg);
 void build_system(const IBMSolver *s, const std::vector<double> *D[3], const std::vector<double> *C[3],
                   const std::vector<double> *Km[3], const std::vector<double> *Kp[3], const int n[3],
                   const IBMSolverBCType bct[3][2], int sk, SysData &out);

@@ -198,9 +198,10 @@ static void wait_epoch(hipStream_t st, unsigned *ep, unsigned e) {
             const hipError_t q = hipStreamQuery(st);
             if (q == hipSuccess) {
                 if (__atomic_load_n(ep, __ATOMIC_ACQUIRE) == e) break;
-                fprintf(stderr, "error: Krylov scalars were not published (epoch %u, expected %u)\n",
-                        __atomic_load_n(ep, __ATOMIC_ACQUIRE), e);
-                abort();
+                char buf[120];
+                snprintf(buf, sizeof buf, "error: Krylov scalars were not published (epoch %u, expected %u)",
+                         __atomic_load_n(ep, __ATOMIC_ACQUIRE), e);
+                case_error(buf);
             }
             if (q != hipErrorNotReady) HIP_CHECK(q);
         }
@@ -1141,9 +1142,8 @@ static void report_solve(IBMSolver *s, int sk, double rel, int it) {
     s->final_norm[sk] = rel;
     s->final_iter[sk] = it;
     if (!std::isfinite(rel)) {
-        if (sk < 3) fprintf(stderr, "error: floating pointer error raised in u%d_star\n", sk + 1);
-        else fprintf(stderr, "error: floating pointer error raised in p_prime\n");
-        abort();
+        if (sk < 3) case_error("error: floating pointer error raised in u" + std::to_string(sk + 1) + "_star");
+        case_error("error: floating pointer error raised in p_prime");
     }
     if (rel > s->tol) {
         if (sk < 3) fprintf(stderr, "warning: u%d_star did not converge\n", sk + 1);
@@ -1275,9 +1275,10 @@ static void momentum_concurrent(IBMSolver *s, F &&norms) {
                 if ((n & 1023) == 0) {
                     const hipError_t q = hipStreamQuery(ln[c].st);
                     if (q == hipSuccess && __atomic_load_n(&ln[c].pub_h->epoch, __ATOMIC_ACQUIRE) != ep[c]) {
-                        fprintf(stderr, "error: Krylov scalars were not published (epoch %u, expected %u)\n",
-                                __atomic_load_n(&ln[c].pub_h->epoch, __ATOMIC_ACQUIRE), ep[c]);
-                        abort();
+                        char buf[120];
+                        snprintf(buf, sizeof buf, "error: Krylov scalars were not published (epoch %u, expected %u)",
+                                 __atomic_load_n(&ln[c].pub_h->epoch, __ATOMIC_ACQUIRE), ep[c]);
+                        case_error(buf);
                     }
                     if (q != hipSuccess && q != hipErrorNotReady) HIP_CHECK(q);
                 }
@@ -1397,7 +1398,9 @@ static void momentum_refine(IBMSolver *s, const double bn[3]) {
     int sweeps[3] = {0, 0, 0};
     bool active[3], fallback[3] = {false, false, false};
     for (int l = 0; l < 3; l++) {
-        active[l] = bn[l] > 0;
+        // a zero right-hand side has the zero solution; a non-finite one
+        // stays active, so its residual reports the error
+        active[l] = bn[l] != 0;
         if (!active[l]) {
             launch_zero<double>(s->stream, d, s->us[l].p);
             report_solve(s, l, 0.0, 0);

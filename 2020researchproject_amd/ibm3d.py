@@ -37,6 +37,17 @@ FIELD = {name: i for i, name in enumerate(FIELDS)}
 PREC_FP64, PREC_MIXED, PREC_MIXED_KRYLOV = 0, 1, 2
 
 INIT_FUNC = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_double)
+
+
+# include/ibm3d_serve.h
+class ServeInit(C.Structure):
+    _fields_ = [("u1", C.c_double), ("u2", C.c_double), ("u3", C.c_double), ("p", C.c_double),
+                ("amp", C.c_double), ("seed", C.c_ulonglong)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("state", C.c_int), ("t", C.c_double * 3), ("iters", C.c_int * 4), ("max_div", C.c_double),
+                ("device_bytes", C.c_double), ("error", C.c_char * 256)]
 BC_FUNC = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_double, C.c_double)
 
 _vp = C.c_void_p
@@ -97,8 +108,11 @@ _OPTIONAL = {
     "IBMSolver_estimate_device_bytes": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "IBMServer_new": (_vp, [C.c_double, C.c_int]),
     "IBMServer_submit": (C.c_int, [_vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, _vp]),
+    "IBMServer_submit_solver": (C.c_int, [_vp, _vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
     "IBMServer_wait": (None, [_vp]),
     "IBMServer_job_times": (C.c_int, [_vp, C.c_int, _dp]),
+    "IBMServer_job_result": (C.c_int, [_vp, C.c_int, C.c_void_p]),
+    "IBMServer_failed_jobs": (C.c_int, [_vp]),
     "IBMServer_peak_bytes": (C.c_double, [_vp]),
     "IBMServer_destroy": (None, [_vp]),
 }

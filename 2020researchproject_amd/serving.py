@@ -3,16 +3,19 @@
 The request loop itself is C++ (csrc/serve.cpp, include/ibm3d_serve.h):
 admission by device-memory footprint (first fit in arrival order), one
 worker thread and one HIP stream per running case, cases overlapping on the
-GPU. This module builds the seeded stream of the mixed-size workload (lid-
-driven cavities and analytic-sphere cases arriving as a Poisson process),
-submits each case at its arrival time, and reports time-to-solution
-percentiles and aggregate cell-steps/s. There is no counterpart in the
-reference, whose drivers run one case per MPI job.
+GPU. The submitting thread configures each case's solver (IBMSolver_new and
+the IBMSolver_set_* calls, host work only) and hands it over with
+IBMServer_submit_solver; the worker assembles it, sets the initial flow, runs
+the steps and records the results in native code, so no Python runs on the
+workers and a case error fails only its job. This module builds the seeded
+stream of the mixed-size workload (lid-driven cavities and analytic-sphere
+cases arriving as a Poisson process), submits each case at its arrival time,
+and reports time-to-solution percentiles and aggregate cell-steps/s. There is
+no counterpart in the reference, whose drivers run one case per MPI job.
 """
 from __future__ import annotations
 
 import ctypes as C
-import threading
 import time
 from dataclasses import dataclass, field
 
@@ -20,9 +23,6 @@ import numpy as np
 
 from . import cases
 from . import ibm3d as I
-
-SETUP_FUNC = C.CFUNCTYPE(C.c_void_p, C.c_void_p)
-DONE_FUNC = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
 
 GRIDS = ((32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 64, 64))
 
@@ -61,6 +61,8 @@ def make_stream(n: int = 64, mean_interarrival: float = 0.050, seed: int = 5, st
 
 
 def build_case(spec: CaseSpec, lib=None) -> I.Solver:
+    """The case built and initialised in this process (the reference run a
+    served case is compared with)."""
     s = I.Solver(lib)
     nx, ny, nz = spec.grid
     if spec.kind == "cavity":
@@ -70,10 +72,27 @@ def build_case(spec: CaseSpec, lib=None) -> I.Solver:
     return s
 
 
+def configure_case(spec: CaseSpec, lib=None) -> tuple[I.Solver, I.ServeInit]:
+    """The case's solver configured but not assembled, and its initial flow,
+    for IBMServer_submit_solver."""
+    s = I.Solver(lib)
+    nx, ny, nz = spec.grid
+    if spec.kind == "cavity":
+        init = cases.setup_cavity(s, Re=spec.Re, tol=spec.tol, precision=spec.precision, shape=(nx, ny, nz),
+                                  assemble=False)
+    elif spec.kind == "sphere":
+        init = cases.setup_sphere(s, nx, ny, nz, Re=spec.Re, tol=spec.tol, precision=spec.precision,
+                                  assemble=False)
+    else:
+        raise ValueError(f"unknown case kind {spec.kind!r}")
+    return s, init
+
+
 @dataclass
 class JobResult:
-    spec: CaseSpec
+    spec: CaseSpec | None
     job: int
+    state: int = 0             # 0 queued, 1 running, 2 done, 3 failed
     submit: float = -1.0
     admit: float = -1.0
     finish: float = -1.0
@@ -96,50 +115,57 @@ class Server:
         nx, ny, nz = spec.grid
         return self.lib.IBMSolver_estimate_device_bytes(nx, ny, nz, spec.precision, 0)
 
-    def submit(self, spec: CaseSpec, keep_fields: bool = False) -> JobResult:
-        res = JobResult(spec, -1)
-        box: dict = {}
-
-        def _setup(_user):
-            try:
-                s = build_case(spec, self.lib)
-                res.device_bytes = s.device_bytes()
-                box["solver"] = s
-                return s.detach()
-            except Exception as e:  # reported with the job
-                res.error = repr(e)
-                return None
-
-        def _done(h, _user):
-            s = box.get("solver")
-            if s is None or not h:
-                return
-            s.h = h  # borrowed until this callback returns; the server destroys it
-            try:
-                res.iters = s.last_iters().tolist()
-                res.max_div = s.max_div()
-                if keep_fields:
-                    res.fields = {n: s.get(n) for n in ("U1", "U2", "U3", "P")}
-            except Exception as e:
-                res.error = repr(e)
-            finally:
-                s.h = None
-
-        cs, cd = SETUP_FUNC(_setup), DONE_FUNC(_done)
-        res.job = self.lib.IBMServer_submit(self.h, self.estimate_bytes(spec), spec.steps,
-                                            C.cast(cs, C.c_void_p), C.cast(cd, C.c_void_p), None)
+    def submit_solver(self, s: I.Solver, init: I.ServeInit | None, bytes_: float, steps: int,
+                      keep_fields: bool = False, res: JobResult | None = None) -> JobResult:
+        """Hands the configured, unassembled solver `s` to the server (which
+        owns it from here on) as a job of `steps` steps."""
+        res = res or JobResult(None, -1)
+        shape = (s.Nx + 4, s.Ny + 4, s.Nz + 4) if keep_fields else None
+        arrays = [np.zeros(shape) for _ in range(4)] if keep_fields else []
+        ptrs = (C.c_void_p * 4)(*[a.ctypes.data for a in arrays]) if keep_fields else None
+        init_ref = C.byref(init) if init is not None else None
+        h = s.detach()
+        res.job = self.lib.IBMServer_submit_solver(self.h, h, float(bytes_), int(steps), init_ref, ptrs)
         if res.job < 0:
-            raise ValueError(f"case does not fit the budget: {spec}")
-        self._keep[res.job] = (cs, cd, box)
+            self.lib.IBMSolver_destroy(h)
+            raise ValueError(f"case does not fit the budget ({bytes_:.3g} bytes)")
+        self._keep[res.job] = (res, arrays, ptrs, init)
         return res
+
+    def submit(self, spec: CaseSpec, keep_fields: bool = False) -> JobResult:
+        s, init = configure_case(spec, self.lib)
+        return self.submit_solver(s, init, self.estimate_bytes(spec), spec.steps, keep_fields,
+                                  JobResult(spec, -1))
 
     def wait(self):
         self.lib.IBMServer_wait(self.h)
+
+    def result(self, job: int) -> I.ServeResult:
+        out = I.ServeResult()
+        self.lib.IBMServer_job_result(self.h, job, C.byref(out))
+        return out
+
+    def collect(self, res: JobResult) -> JobResult:
+        """Copies the finished job's outcome into res."""
+        r = self.result(res.job)
+        res.state = r.state
+        res.submit, res.admit, res.finish = tuple(r.t)
+        res.iters = list(r.iters)
+        res.max_div = r.max_div
+        res.device_bytes = r.device_bytes
+        res.error = r.error.decode() or None
+        _, arrays, _, _ = self._keep.get(res.job, (None, [], None, None))
+        if arrays and r.state == 2:
+            res.fields = dict(zip(("U1", "U2", "U3", "P"), arrays))
+        return res
 
     def times(self, job: int):
         out = (C.c_double * 3)()
         state = self.lib.IBMServer_job_times(self.h, job, out)
         return state, tuple(out)
+
+    def failed_jobs(self) -> int:
+        return self.lib.IBMServer_failed_jobs(self.h)
 
     def peak_bytes(self) -> float:
         return self.lib.IBMServer_peak_bytes(self.h)
@@ -157,11 +183,12 @@ class Server:
 
 
 def run_stream(specs: list[CaseSpec], budget_bytes: float, max_active: int = 4, keep_fields: bool = False,
-               lib=None) -> dict:
+               lib=None, allow_failures: bool = False) -> dict:
     """Submits every case at its arrival time (relative to the first) and
     waits for all of them. Returns per-job results and the summary:
     time-to-solution = completion - submission, aggregate Mcell-steps/s =
-    sum(cells * steps) / (last completion - first submission)."""
+    sum(cells * steps) / (last completion - first submission), the cell-steps
+    of the cases that finished. A failed case raises unless allow_failures."""
     srv = Server(budget_bytes, max_active, lib)
     results: list[JobResult] = []
     t0 = time.perf_counter()
@@ -172,15 +199,15 @@ def run_stream(specs: list[CaseSpec], budget_bytes: float, max_active: int = 4, 
         results.append(srv.submit(spec, keep_fields))
     srv.wait()
     for r in results:
-        _, (r.submit, r.admit, r.finish) = srv.times(r.job)
+        srv.collect(r)
     peak = srv.peak_bytes()
     srv.close()
-    errors = [r.error for r in results if r.error]
-    if errors:
+    errors = [r.error for r in results if r.state != 2]
+    if errors and not allow_failures:
         raise RuntimeError(f"{len(errors)} case(s) failed: {errors[:3]}")
     tts = np.array([r.finish - r.submit for r in results])
     span = max(r.finish for r in results) - min(r.submit for r in results)
-    work = sum(r.spec.cells * r.spec.steps for r in results)
+    work = sum(r.spec.cells * r.spec.steps for r in results if r.state == 2)
     return {
         "results": results,
         "p50_s": float(np.percentile(tts, 50)),

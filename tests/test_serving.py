@@ -4,8 +4,10 @@ A small mixed stream (cavities and spheres of several sizes, arriving over
 time) runs through the native request loop with a budget that lets only
 some cases run at once. Checks: every case finishes, admitted footprints
 stay within the budget, each case's device memory stays within the
-footprint estimate used for admission, and a case's fields are bit-for-bit
-the same when it runs alone and when it runs in the mix."""
+footprint estimate used for admission, a case's fields are bit-for-bit
+the same when it runs alone and when it runs in the mix, and a case that
+fails (a non-finite residual) fails alone: the cases beside it finish with
+the fields of the same cases built and run in this process."""
 from __future__ import annotations
 
 import numpy as np
@@ -50,6 +52,52 @@ def test_serving_same_result_alone_and_in_mix(I):
         m = next(r for r in mix["results"] if r.spec is target).fields
         for n in ("U1", "U2", "U3", "P"):
             assert np.array_equal(a[n], m[n], equal_nan=True), (target, n)
+
+
+def test_failing_job_fails_alone(I):
+    """A mix of four cases and one that blows up (initial u1 = NaN, so the
+    first u1 solve has a non-finite residual). The failing job reports the
+    reference's message and state 3; the others finish, bit-for-bit equal
+    to the same case built with cases.setup_* (its Python initial-flow path)
+    and run in this process, which also pins the native seeded
+    perturbation of IBMServeInit against cases.uniform01."""
+    from importlib import import_module
+    S = import_module("2020researchproject_amd.serving")
+    P = I.PREC_MIXED_KRYLOV
+    good = [
+        S.CaseSpec("cavity", (32, 32, 32), 400.0, 6, 0.0, 1e-10, P),
+        S.CaseSpec("sphere", (48, 24, 24), 300.0, 5, 0.0, 1e-10, P),
+        S.CaseSpec("cavity", (24, 32, 16), 900.0, 7, 0.0, 1e-10, P),
+        S.CaseSpec("cavity", (16, 16, 16), 100.0, 4, 0.0, 1e-10, I.PREC_FP64),
+    ]
+    srv = S.Server(4e9, 3)
+    res = []
+    for c in good[:2]:
+        res.append(srv.submit(c, keep_fields=True))
+    bad_spec = S.CaseSpec("cavity", (32, 32, 32), 400.0, 6, 0.0, 1e-10, P)
+    s, init = S.configure_case(bad_spec)
+    init.u1 = float("nan")
+    bad = srv.submit_solver(s, init, srv.estimate_bytes(bad_spec), bad_spec.steps, keep_fields=True)
+    for c in good[2:]:
+        res.append(srv.submit(c, keep_fields=True))
+    srv.wait()
+    srv.collect(bad)
+    assert bad.state == 3 and "floating pointer error raised in u1_star" in bad.error, (bad.state, bad.error)
+    assert bad.fields is None
+    for r in res:
+        srv.collect(r)
+    assert srv.failed_jobs() == 1
+    srv.close()
+    for r in res:
+        assert r.state == 2 and r.error is None, (r.spec, r.error)
+        here = S.build_case(r.spec)
+        here.iterate(r.spec.steps)
+        assert list(here.last_iters()) == r.iters, r.spec
+        for n in ("U1", "U2", "U3", "P"):
+            # solid cells hold NaN in both (the reference's solid-cell value)
+            assert np.array_equal(here.get(n), r.fields[n], equal_nan=True), (r.spec, n)
+        here.close()
+    print("failing job:", bad.error.strip(), flush=True)
 
 
 @pytest.mark.parametrize("precision", [0, 1, 2])

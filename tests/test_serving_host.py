@@ -64,3 +64,49 @@ def test_admission_by_footprint_first_fit(gpu_lib):
     # the admission times (taken in arrival order) are checked, not `order`.
     assert admitted[0] <= admitted[2] <= admitted[5] < admitted[1]
     assert sorted(order) == list(range(len(sizes)))
+
+
+def _configured(I, n=(8, 8, 8), precision=0, solver=None, periodic_x=None):
+    """An unassembled cavity-like solver; periodic_x: None, "both" or "west"."""
+    import numpy as np
+    s = I.Solver()
+    nx, ny, nz = n
+    s.set_grid(np.linspace(0, 1, nx + 1), np.linspace(0, 1, ny + 1), np.linspace(0, 1, nz + 1))
+    s.set_params(100.0, 0.01)
+    s.set_bc(I.DIR_NORTH | I.DIR_EAST | I.DIR_WEST | I.DIR_SOUTH | I.DIR_DOWN | I.DIR_UP, I.BC_STATIONARY_WALL)
+    if periodic_x == "both":
+        s.set_bc(I.DIR_WEST | I.DIR_EAST, I.BC_ALL_PERIODIC)
+    elif periodic_x == "west":
+        s.set_bc(I.DIR_WEST, I.BC_ALL_PERIODIC)
+    s.set_linear_solver(solver if solver is not None else I.SOLVER_BiCGSTAB, I.PRECOND_AMG, 1e-10)
+    s.set_precision(precision)
+    return s
+
+
+def test_failed_jobs_fail_alone(I, gpu_lib, capfd):
+    """Jobs whose input IBMSolver_assemble rejects fail with the reference's
+    message while the server (and this process) goes on: the error is a job
+    state, its budget is released, and a job without a grid fails the same
+    way. All four fail before any device work, so no GPU is needed."""
+    from importlib import import_module
+    serving = import_module("2020researchproject_amd.serving")
+    srv = serving.Server(1000.0, 2, gpu_lib._name)
+    bad = [
+        (_configured(I, periodic_x="west"), "Inconsistent periodic boundary condition"),
+        (_configured(I, n=(9, 8, 8), periodic_x="both"), "periodic x axis with an odd cell count (9)"),
+        (_configured(I, precision=I.PREC_MIXED, solver=I.SOLVER_GMRES), "IBM3D_PREC_MIXED supports only"),
+        (I.Solver(), "IBMSolver_set_grid must precede IBMSolver_assemble"),
+    ]
+    res = [srv.submit_solver(s, None, 400.0, 3) for s, _ in bad]
+    srv.wait()
+    for r, (_, msg) in zip(res, bad):
+        srv.collect(r)
+        assert r.state == 3 and msg in r.error, (r.state, r.error)
+        assert 0 <= r.submit <= r.admit <= r.finish
+    assert srv.failed_jobs() == 4
+    # 400 + 400 fit the budget of 1000 together, a third did not
+    assert srv.peak_bytes() == 800.0
+    srv.close()
+    out = capfd.readouterr()
+    assert "Inconsistent periodic boundary condition" in out.out
+    assert "odd cell count (9)" in out.err
