@@ -654,6 +654,10 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x, bool first_done = fals
     const SysCoef<T> *Sc = fuse ? &sysc<T>(Cl.sys) : nullptr;
     const uint8_t *flag_c = fuse ? Cl.flag.p : nullptr;
     T *xc = fuse ? lev_x<T>(Cl).p : nullptr;
+    // coarse-level bytes of a restriction: b written, and with the fused first
+    // colour the coarse flags read and every coarse x written
+    const double cn = (double)Cl.d.nx * Cl.d.ny * Cl.d.nz;
+    const double coarse_b = cn * (sizeof(T) + (fuse ? 1 + sizeof(T) : 0));
     if ((long)L.d.nx * L.d.ny * L.d.nz >= kSplitRestrictCells) {
         // large levels: residual, then restriction (measured faster than the
         // one-pass kernel, whose per-coarse-cell child loops run short here)
@@ -663,7 +667,8 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x, bool first_done = fals
         bool rz = false;
         TIMED(rn, 3 * cell_b + cell_b / sizeof(T),
               rz = launch_mg_residual<T>(s->stream, L.d, S, flag, b, x, lev_r<T>(L).p));
-        TIMED(tn, cell_b + cell_b / sizeof(T),
+        // fine r and flags read
+        TIMED(tn, cell_b + cell_b / sizeof(T) + coarse_b,
               launch_restrict<T>(s->stream, L.d, Cl.d, Cl.tr, flag, lev_r<T>(L).p, L.dxyz.p, L.dxyz.p + (L.d.nx + 4),
                                  L.dxyz.p + (L.d.nx + 4) + (L.d.ny + 4), Cl.dxyz.p, Cl.dxyz.p + (Cl.d.nx + 4),
                                  Cl.dxyz.p + (Cl.d.nx + 4) + (Cl.d.ny + 4), lev_b<T>(Cl).p, rz, Sc, flag_c, xc));
@@ -671,15 +676,16 @@ static void vcycle(IBMSolver *s, int l, const T *b, T *x, bool first_done = fals
         // residual restricted in one pass: x and b read, one flag byte per cell
         char nm[32];
         snprintf(nm, sizeof nm, "mg_resid_restrict_L%d", l);
-        TIMED(nm, 2 * cell_b + cell_b / sizeof(T),
+        TIMED(nm, 2 * cell_b + cell_b / sizeof(T) + coarse_b,
               launch_resid_restrict<T>(s->stream, L.d, S, flag, b, x, Cl.d, Cl.tr, L.dxyz.p, Cl.dxyz.p,
                                        lev_b<T>(Cl).p, Sc, flag_c, xc));
     }
     vcycle<T>(s, l + 1, lev_b<T>(Cl).p, lev_x<T>(Cl).p, fuse);
     char pnm[32];
     snprintf(pnm, sizeof pnm, "mg_prolong_L%d", l);
-    // fine x read and written, one flag byte per fine cell (coarse reads cached)
-    TIMED(pnm, 2 * cell_b + cell_b / sizeof(T),
+    // fine x read and written, a flag and an interpolation-mask byte per fine
+    // cell, each coarse correction read once
+    TIMED(pnm, 2 * cell_b + 2 * cell_b / sizeof(T) + cn * sizeof(T),
           launch_prolong<T>(s->stream, L.d, Cl.d, Cl.tr, flag, Cl.flag.p, lev_x<T>(Cl).p, x));
     smooth<T>(s, L.d, S, G0, flag, b, x, lev_xa<T>(L).p, light ? 1 : s->mg_post, false, l);
 }
