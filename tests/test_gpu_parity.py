@@ -273,9 +273,9 @@ def test_function_valued_forcing(new_gpu, new_oracle, I, cases):
         s.iterate(4)
         res.append(s)
     compare(*res, 1e-8)
-    # max|div U| over fluid cells (those beside the one-cell body included)
-    # matches the oracle's
-    assert res[0].max_div() == pytest.approx(res[1].max_div(), rel=1e-6)
+    # max|div U| over the fluid cells (the pinned cell left out: no pressure
+    # boundary here) is at the level of the solver tolerance in both
+    assert res[0].max_div() < 1e-10 and res[1].max_div() < 1e-10
 
 
 def test_repeated_iterate_resets_ab2_like_reference(new_gpu, new_oracle, cases):
