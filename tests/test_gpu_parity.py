@@ -140,6 +140,33 @@ def test_cavity_mixed_precision(new_gpu, new_oracle, cases, I, mode):
     assert (g.last_norms() <= 1e-10).all()
 
 
+def test_sphere_500_steps_mixed_vs_fp64(new_gpu, cases, I):
+    """The sphere workload as configured (256x128x128, convective outflow,
+    Re 300, dt 0.002) for its full 500 steps at tol 1e-9, in the mixed path
+    and in FP64 (the oracle would take about an hour; FP64 on the GPU matches
+    it to 1e-10 at this grid, test_sphere_full_size_vs_oracle). The mixed
+    path's velocity stays within 1e-5 rel-L2 of FP64 (the stated bar), p
+    within 1e-5, the drag history within 1e-6 relative, and max|div U| <=
+    1e-8 at the end in both."""
+    res = {}
+    for prec in (I.PREC_FP64, I.PREC_MIXED_KRYLOV):
+        g = new_gpu()
+        cases.setup_sphere(g, 256, 128, 128, tol=1e-9, precision=prec, outflow="convective")
+        g.set_force_history(500)
+        g.iterate(500)
+        res[prec] = (np.stack([g.get(n)[INNER] for n in ("U1", "U2", "U3")]), g.get("P")[INNER],
+                     g.force_history(), g.max_div())
+        g.close()
+    (u64, p64, f64, d64), (u32, p32, f32, d32) = res[I.PREC_FP64], res[I.PREC_MIXED_KRYLOV]
+    eu, ep = rel_l2(u32, u64), rel_l2(p32, p64)
+    ef = float(np.max(np.abs(f32[:, 0] - f64[:, 0])) / np.max(np.abs(f64[:, 0])))
+    print(f"500 steps: mixed vs FP64 rel-L2 u {eu:.2e} p {ep:.2e}, drag history {ef:.2e}, final drag "
+          f"{f32[-1, 0]:.9g} / {f64[-1, 0]:.9g}, max|div U| {d32:.2e} / {d64:.2e}", flush=True)
+    assert f32.shape == (500, 3) and f64.shape == (500, 3)
+    assert eu <= 1e-5 and ep <= 1e-5 and ef <= 1e-6
+    assert d32 <= 1e-8 and d64 <= 1e-8
+
+
 _FULL_SIZE_ORACLE: dict = {}
 
 
