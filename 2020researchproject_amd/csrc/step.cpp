@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <sched.h>
 
 #include "solver.h"
 
@@ -189,6 +190,15 @@ static void update_ghost(IBMSolver *s) {
                               s->ghost_old.p, s->u[0].p, s->u[1].p, s->u[2].p, s->p.p));
 }
 
+// One poll of a host wait on device-published values: a pause for the first
+// ~2000 polls (about 0.1 ms), then a yield of the core, so that waits of
+// many solvers on different threads (the serving loop) leave the cores to
+// the threads that have work.
+static inline void wait_pause(long n) {
+    if (n < 2048) __builtin_ia32_pause();
+    else sched_yield();
+}
+
 // Waits until a kernel enqueued on `st` has published epoch e at *ep (host
 // memory), polling the stream now and then so that an error or a lost
 // publication cannot make the wait endless.
@@ -205,7 +215,7 @@ static void wait_epoch(hipStream_t st, unsigned *ep, unsigned e) {
             }
             if (q != hipErrorNotReady) HIP_CHECK(q);
         }
-        __builtin_ia32_pause();
+        wait_pause(n);
     }
 }
 static void wait_published(hipStream_t st, KPub *pub_h, unsigned e) { wait_epoch(st, &pub_h->epoch, e); }
@@ -1298,7 +1308,7 @@ static void momentum_concurrent(IBMSolver *s, F &&norms) {
             if (act[c]) { enqueue_iteration(c); fly[c] = true; }
         }
         if (!any) break;
-        __builtin_ia32_pause();
+        wait_pause(n);
     }
     // true residuals of the final iterates
     for (int c = 0; c < 3; c++) {

@@ -332,7 +332,8 @@ def print_stats(tag: str, res) -> None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200,
+                    help="timed steps (200: p50/p99 of the per-step times over 200 samples)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=str, default="256x128x128")
     ap.add_argument("--tol", type=float, default=1e-9)
