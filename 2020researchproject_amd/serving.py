@@ -129,7 +129,9 @@ class Server:
         if res.job < 0:
             self.lib.IBMSolver_destroy(h)
             raise ValueError(f"case does not fit the budget ({bytes_:.3g} bytes)")
-        self._keep[res.job] = (res, arrays, ptrs, init)
+        # the Solver object stays referenced: it holds the ctypes callbacks
+        # (function-valued boundary data) the worker's native code may call
+        self._keep[res.job] = (res, arrays, ptrs, init, s)
         return res
 
     def submit(self, spec: CaseSpec, keep_fields: bool = False) -> JobResult:
@@ -154,7 +156,7 @@ class Server:
         res.max_div = r.max_div
         res.device_bytes = r.device_bytes
         res.error = r.error.decode() or None
-        _, arrays, _, _ = self._keep.get(res.job, (None, [], None, None))
+        arrays = self._keep[res.job][1] if res.job in self._keep else []
         if arrays and r.state == 2:
             res.fields = dict(zip(("U1", "U2", "U3", "P"), arrays))
         return res
