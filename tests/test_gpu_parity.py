@@ -167,6 +167,18 @@ def test_sphere_500_steps_mixed_vs_fp64(new_gpu, cases, I):
     assert d32 <= 1e-8 and d64 <= 1e-8
 
 
+@pytest.mark.parametrize("precision", [0, 2], ids=["fp64", "mixed"])
+def test_poiseuille_steady_state_on_gpu(new_gpu, I, precision):
+    """The closed-form pin of the oracle's full step, on the GPU: the forced
+    channel settles to u = 1 - z^2 up to the wall rows' discrete error h^2/4
+    (test_oracle.py::test_poiseuille_steady_state_second_order)."""
+    from test_oracle import poiseuille_error
+    for nz, h in ((16, 0.125), (32, 0.0625)):
+        g = new_gpu()
+        g.set_precision(precision)
+        assert poiseuille_error(I, g, nz) == pytest.approx(h * h / 4, rel=1e-6)
+
+
 _FULL_SIZE_ORACLE: dict = {}
 
 

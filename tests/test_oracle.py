@@ -224,6 +224,74 @@ def test_convection_exact_for_linear_field(I, new_oracle, cases):
     assert np.abs(s.get("N1")[2:-2, 2:-2, 2:-2]).max() < 1e-14
 
 
+def test_convection_conserves_momentum_in_a_periodic_box(I, new_oracle, cases):
+    """calc_N is a flux difference: every face flux U u_face enters the two
+    cells beside it with opposite signs, so on a periodic box the volume
+    integral of N vanishes for any velocity field, here a seeded random one
+    on a stretched grid. A face value interpolated differently from the two
+    sides, a wrong spacing in one term or a dropped face breaks it."""
+    s = new_oracle()
+    bind_oracle_hooks(s.lib)
+    xf = np.cumsum(np.concatenate([[0.0], 0.5 + np.linspace(0, 1, 10) ** 2]))
+    yf = cases.uniform_faces(0.0, 2.0, 8)
+    zf = np.cumsum(np.concatenate([[0.0], 1.0 + 0.5 * np.sin(np.arange(6))]))
+    box(I, s, xf, yf, zf, Re=50.0, dt=0.01, bc=lambda s: s.set_bc(0x3f, I.BC_ALL_PERIODIC))
+    s.init_flow_const(0, 0, 0, 0)
+    rng = np.random.default_rng(11)
+    for n in ("U1", "U2", "U3"):
+        s.set(n, rng.uniform(-1, 1, s.shape))
+    s.lib.oracle_update_outer(s.h)  # periodic images in the outer layers
+    s.lib.oracle_calc_N(s.h)
+    _, _, _, dx, dy, dz = coords(s)
+    V = dx[2:-2, None, None] * dy[None, 2:-2, None] * dz[None, None, 2:-2]
+    for n in ("N1", "N2", "N3"):
+        N = s.get(n)[2:-2, 2:-2, 2:-2]
+        scale = np.sum(np.abs(N) * V)
+        assert scale > 1.0
+        assert abs(np.sum(N * V)) <= 1e-13 * scale, n
+
+
+def poiseuille_error(I, s, nz, steps=500, dt=0.02):
+    """Channel between stationary walls at z = -1, 1 (periodic x, y), Re 1,
+    driven by the body force f = 2 / Re: the steady solution is u = 1 - z^2
+    (the reference's channel set-up, src/channel.c:18, 102-104). Returns the
+    largest error at the cell centres after the flow has settled."""
+    f4 = cases_uniform(0.0, 1.0, 4)
+    zf = cases_uniform(-1.0, 1.0, nz)
+    s.set_grid(f4, f4, zf)
+    s.set_params(1.0, dt)
+    s.set_bc(I.DIR_EAST | I.DIR_WEST | I.DIR_NORTH | I.DIR_SOUTH, I.BC_ALL_PERIODIC)
+    s.set_bc(I.DIR_DOWN | I.DIR_UP, I.BC_STATIONARY_WALL)
+    s.set_ext_force(2.0, 0.0, 0.0)
+    s.set_obstacle(None)
+    s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, 1e-10)
+    s.assemble()
+    s.init_flow_const(0, 0, 0, 0)
+    s.iterate(steps)
+    zc = (zf[1:] + zf[:-1]) / 2
+    u = s.get("U1")[2:-2, 2:-2, 2:-2]
+    assert np.abs(s.get("U2")[2:-2, 2:-2, 2:-2]).max() < 1e-9
+    assert np.abs(s.get("U3")[2:-2, 2:-2, 2:-2]).max() < 1e-9
+    return float(np.abs(u - (1 - zc ** 2)[None, None, :]).max())
+
+
+def cases_uniform(a, b, n):
+    return np.array([a + (b - a) * i / n for i in range(n + 1)])
+
+
+def test_poiseuille_steady_state_second_order(I, new_oracle):
+    """The full step (momentum RHS with the body force, the wall rows, the
+    Helmholtz and pressure solves, the projection) settles to the parabolic
+    profile, with the second-order error of the wall rows: the error falls by
+    about 4x when the cells halve."""
+    e16 = poiseuille_error(I, new_oracle(), 16)
+    e32 = poiseuille_error(I, new_oracle(), 32)
+    # the discrete steady state is off by h^2 / 4 (h the cell width): the
+    # wall row mirrors the first cell linearly through the zero face value
+    assert e16 == pytest.approx(0.125 ** 2 / 4, rel=1e-6)
+    assert e32 == pytest.approx(0.0625 ** 2 / 4, rel=1e-6)
+
+
 # ---------------------------------------------------------------------------
 # a10: outer ghost layers  <- src/ibm3d_fracstep.c:855-1906
 # ---------------------------------------------------------------------------
