@@ -390,7 +390,7 @@ def main() -> int:
         "iters_last_step": mixed["iters"],
         # drag and lift coefficients (Cd, Cl_y, Cl_z) after each timed step
         "cd_history": [round(c[0], 6) for c in mixed["coef_hist"]],
-        "cl_history": [[round(c[1], 6), round(c[2], 6)] for c in mixed["coef_hist"]],
+        "cl_history": [[float(f"{c[1]:.6g}"), float(f"{c[2]:.6g}")] for c in mixed["coef_hist"]],
     }
     rf = roofline(mixed["stats"])
     if rf:
