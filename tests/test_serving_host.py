@@ -110,3 +110,19 @@ def test_failed_jobs_fail_alone(I, gpu_lib, capfd):
     out = capfd.readouterr()
     assert "Inconsistent periodic boundary condition" in out.out
     assert "odd cell count (9)" in out.err
+
+
+def test_native_submission_refusals(I, gpu_lib):
+    """IBMServer_submit_solver refuses a NULL solver and a footprint above the
+    budget (-1, the solver stays the caller's); IBMServer_job_result of an
+    unknown job is -1."""
+    import ctypes as C
+    srv = gpu_lib.IBMServer_new(100.0, 1)
+    assert gpu_lib.IBMServer_submit_solver(srv, None, 10.0, 1, None, None) == -1
+    s = _configured(I)
+    assert gpu_lib.IBMServer_submit_solver(srv, s.h, 101.0, 1, None, None) == -1
+    res = I.ServeResult()
+    assert gpu_lib.IBMServer_job_result(srv, 0, C.byref(res)) == -1
+    assert gpu_lib.IBMServer_failed_jobs(srv) == 0
+    gpu_lib.IBMServer_destroy(srv)
+    s.close()  # still the caller's: destroyed here, unassembled
