@@ -246,13 +246,17 @@ def setup_sphere(s: I.Solver, nx: int = 256, ny: int = 128, nz: int = 128, Re: f
 
 
 # ---------------------------------------------------------------------------
-# LES channel at Re_tau = 180 (the reference's src/channel.c:47-60 set-up with
-# y as the wall-normal axis): periodic x and z, stationary walls at y = +-1,
-# y_j = -cos(pi j / Ny), constant mean pressure gradient f = 1 (u_tau = 1,
-# delta = 1, nu = 1 / 180).
+# LES channel at Re_tau = 180 in the reference's units (src/channel.c:13-18,
+# 47-60, 102-104, with y as the wall-normal axis): periodic x and z,
+# stationary walls at y = +-1, y_j = -cos(pi j / Ny), velocity scaled by the
+# laminar centreline velocity (u = 1 - y^2), Re = 3300 (the classic
+# centreline Reynolds number of the Re_tau = 180 channel), constant mean
+# pressure gradient f = 2 / Re (the one that holds the laminar profile).
 # ---------------------------------------------------------------------------
 CHANNEL_SEED = 3
-CHANNEL_BULK = 15.6  # bulk velocity of the initial parabola, in u_tau
+CHANNEL_RE = 3300.0
+CHANNEL_RETAU = 180.0
+CHANNEL_BULK = 2.0 / 3.0  # bulk velocity of the laminar parabola
 
 
 def channel_faces(nx: int = 192, ny: int = 96, nz: int = 128):
@@ -261,17 +265,30 @@ def channel_faces(nx: int = 192, ny: int = 96, nz: int = 128):
             uniform_faces(0.0, 4 * math.pi / 3, nz))
 
 
-def channel_initial(xc, yc, zc, seed: int = CHANNEL_SEED, bulk: float = CHANNEL_BULK, amp: float = 0.1):
+def channel_initial(xc, yc, zc, seed: int = CHANNEL_SEED, bulk: float = CHANNEL_BULK, amp: float = 0.1,
+                    smooth: int = 4):
     """Laminar parabola u = 1.5 bulk (1 - y^2) plus seeded Gaussian noise
     made solenoidal: the noise is the curl of a vector potential whose
-    components are independent Gaussian draws (one per padded cell, seed)
-    damped by (1 - y^2)^2 toward the walls, by central differences (periodic
-    in x and z), scaled to an RMS of amp * bulk over the interior. Arrays are
-    over the padded cell centres (the reference layout)."""
+    components are independent Gaussian draws (one per padded cell, seed),
+    smoothed by `smooth` passes of the [1, 2, 1] / 4 filter along each axis
+    (periodic in x and z; so that the noise is resolved by the grid rather
+    than a cell-to-cell oscillation) and damped by (1 - y^2)^2 toward the
+    walls; the curl by central differences (periodic in x and z), scaled to
+    an RMS of amp * bulk over the interior. Arrays are over the padded cell
+    centres (the reference layout)."""
     rng = np.random.default_rng(seed)
     X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
     damp = np.clip(1 - Y**2, 0.0, None) ** 2
-    psi = [damp * rng.standard_normal(X.shape) for _ in range(3)]
+
+    def filt(f):
+        for _ in range(smooth):
+            for axis in (0, 2):
+                f = 0.25 * np.roll(f, 1, axis) + 0.5 * f + 0.25 * np.roll(f, -1, axis)
+            g = f.copy()
+            g[:, 1:-1, :] = 0.25 * f[:, :-2, :] + 0.5 * f[:, 1:-1, :] + 0.25 * f[:, 2:, :]
+            f = g
+        return f
+    psi = [damp * filt(rng.standard_normal(X.shape)) for _ in range(3)]
 
     def d(f, axis, c):
         # central difference on the padded array, periodic wrap where it runs
@@ -296,19 +313,21 @@ def setup_channel(s: I.Solver, nx: int = 192, ny: int = 96, nz: int = 128, dt: f
                   tol: float = 1e-9, precision: int = I.PREC_FP64, turb: tuple | None = None,
                   seed: int = CHANNEL_SEED):
     """The LES channel workload: turb = (TURBMODEL_SMAGORINSKY_VD, Cs) for
-    Smagorinsky with van Driest damping at the y walls (A+ = 26, u_tau = 1),
-    (TURBMODEL_VREMAN, c) for Vreman, (TURBMODEL_SMAGORINSKY, Cs) for the
-    reference's model, None for no model."""
+    Smagorinsky with van Driest damping at the y walls (A+ = 26, u_tau =
+    Re_tau / Re, so that y+ = Re_tau y_wall), (TURBMODEL_VREMAN, c) for
+    Vreman, (TURBMODEL_SMAGORINSKY, Cs) for the reference's model, None for
+    no model."""
     xf, yf, zf = channel_faces(nx, ny, nz)
     s.set_grid(xf, yf, zf)
-    s.set_params(180.0, dt)
+    s.set_params(CHANNEL_RE, dt)
     s.set_bc(I.DIR_WEST | I.DIR_EAST | I.DIR_DOWN | I.DIR_UP, I.BC_ALL_PERIODIC)
     s.set_bc(I.DIR_NORTH | I.DIR_SOUTH, I.BC_STATIONARY_WALL)
-    s.set_ext_force(1.0, 0.0, 0.0)
+    s.set_ext_force(2.0 / CHANNEL_RE, 0.0, 0.0)
     if turb is not None:
         model, coef = turb
         if model == I.TURBMODEL_SMAGORINSKY_VD:
-            s.set_turb_model(model, coef, aplus=26.0, u_tau=1.0, walls=I.DIR_NORTH | I.DIR_SOUTH)
+            s.set_turb_model(model, coef, aplus=26.0, u_tau=CHANNEL_RETAU / CHANNEL_RE,
+                             walls=I.DIR_NORTH | I.DIR_SOUTH)
         else:
             s.set_turb_model(model, coef)
     s.set_obstacle(None)
