@@ -477,8 +477,10 @@ static void build_ghosts(IBMSolver *s) {
             G->v64[k].alloc(std::max(ng, 1));
             G->v32[k].alloc(std::max(ng, 1));
         }
-        G->G64 = {ng, G->cell.p, G->src.p, G->w64.p, G->ws64.p, G->srcg.p, {G->v64[0].p, G->v64[1].p}};
-        G->G32 = {ng, G->cell.p, G->src.p, G->w32.p, G->ws32.p, G->srcg.p, {G->v32[0].p, G->v32[1].p}};
+        G->G64 = {ng, G->cell.p, G->src.p, G->w64.p, G->ws64.p, G->srcg.p, {G->v64[0].p, G->v64[1].p},
+                  G->w64.p, G->ws64.p};
+        G->G32 = {ng, G->cell.p, G->src.p, G->w32.p, G->ws32.p, G->srcg.p, {G->v32[0].p, G->v32[1].p},
+                  G->w64.p, G->ws64.p};
     }
 }
 

@@ -16,6 +16,11 @@ struct GhostRows {
     const T *wself;     // ng
     const int *srcg;    // 8 * ng: ghost row of the source cell, -1 for a fluid source
     T *vals[2];         // ng each: ghost values of the last two relaxations
+    // the same weights in FP64: the smoothers' ghost-row relaxation forms its
+    // weighted sum and divides by 1 -/+ w_self in FP64 whatever T is, since a
+    // nearly degenerate stencil (w_self close to 1) would amplify the
+    // rounding of FP32 weights
+    const double *w64, *ws64;
 };
 
 // Device-resident scalars of BiCGSTAB.

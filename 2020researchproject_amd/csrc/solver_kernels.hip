@@ -703,12 +703,12 @@ __device__ inline void ghost_row_in_colour(const Dims &d, const SysCoef<T> &S, c
     }
     T vals[8];
     bool used[8];
-    T wv[8];
+    double wv[8];
 #pragma unroll
     for (int l = 0; l < 8; l++) {
         const long q0 = G.src[8 * (long)s + l];
         const int g = G.srcg[8 * (long)s + l];
-        wv[l] = G.w[8 * (long)s + l];
+        wv[l] = G.w64[8 * (long)s + l];
         used[l] = q0 >= 0;
         const long q = q0 >= 0 ? q0 : c;
         int i, j, k;
@@ -726,17 +726,17 @@ __device__ inline void ghost_row_in_colour(const Dims &d, const SysCoef<T> &S, c
         else if (same && f == F_FLUID) v = diag != T(0) ? (bq + off) / diag : T(0);
         vals[l] = v;
     }
-    T a = 0;
+    double a = 0;
 #pragma unroll
     for (int l = 0; l < 8; l++)
-        if (used[l]) a += wv[l] * vals[l];
-    const T ws = G.wself[s];
+        if (used[l]) a += wv[l] * (double)vals[l];
+    const double ws = G.ws64[s];
     T v;
     if (is_p) {
-        const T dg = T(1) - ws;
-        v = dg != T(0) ? (b[c] + a) / dg : x[c];
+        const double dg = 1.0 - ws;
+        v = dg != 0.0 ? (T)(((double)b[c] + a) / dg) : x[c];
     } else {
-        v = (b[c] - a) / (T(1) + ws);
+        v = (T)(((double)b[c] - a) / (1.0 + ws));
     }
     x[c] = v;
     next[s] = v;
@@ -916,23 +916,23 @@ __global__ void k_ghost_relax(GhostRows<T> G, int is_p, const T *b, T *x, const 
             return;
         }
     }
-    T a = 0;
+    double a = 0;
 #pragma unroll
     for (int l = 0; l < 8; l++) {
         const long c = G.src[8 * (long)s + l];
         if (c < 0) continue;
         const int g = G.srcg[8 * (long)s + l];
         const T v = g < 0 ? x[c] : prev ? prev[g] : T(0);
-        a += G.w[8 * (long)s + l] * v;
+        a += G.w64[8 * (long)s + l] * (double)v;
     }
     const long c = G.cell[s];
-    const T ws = G.wself[s];
+    const double ws = G.ws64[s];
     T v;
     if (is_p) {
-        const T dg = T(1) - ws;
-        v = dg != T(0) ? (b[c] + a) / dg : x[c];
+        const double dg = 1.0 - ws;
+        v = dg != 0.0 ? (T)(((double)b[c] + a) / dg) : x[c];
     } else {
-        v = (b[c] - a) / (T(1) + ws);
+        v = (T)(((double)b[c] - a) / (1.0 + ws));
     }
     x[c] = v;
     next[s] = v;

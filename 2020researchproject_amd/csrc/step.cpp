@@ -1149,7 +1149,7 @@ static int refine_mixed(IBMSolver *s, int sk, const double *b, double *x, int pr
 }
 
 // BiCGSTAB iterations between restarts when the preconditioner runs in FP32
-static const int kMixedRestart = 20;
+static const int kMixedRestart = 40;
 
 // Records the outcome of the solve of system sk and reacts like the reference
 // (src/ibm3d_fracstep.c:363-369, 673-679): a non-finite residual aborts,
