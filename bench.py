@@ -381,7 +381,8 @@ def main() -> int:
         "config": {"workload": f"sphere {nx}x{ny}x{nz} cosine-stretched, Re 300, dt 0.002, uniform inflow, "
                                f"{args.outflow} outflow, slip lateral walls, tol {args.tol:g}",
                    "cells": int(cells),
-                   "precision": ("FP64 BiCGSTAB, FP32 multigrid preconditioner" if args.mixed_mode == 2 else
+                   "precision": ("momentum: FP32 red-black sweeps in FP64 refinement; pressure: FP64 BiCGSTAB, "
+                                 "FP32 multigrid preconditioner" if args.mixed_mode == 2 else
                                  "FP64 defect correction around FP32 multigrid-BiCGSTAB"),
                    "mg_levels": mixed["mg_levels"]},
         "p50_ms": round(float(np.percentile(mixed["step_ms"], 50)), 3),
