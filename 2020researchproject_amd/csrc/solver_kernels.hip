@@ -1734,17 +1734,21 @@ constexpr int kDenseMaxRows = 4096;  // right-hand side staged in LDS
 
 // One wavefront per row of the inverse; four independent partial sums per
 // lane keep several row loads in flight.
+// rows per workgroup (one wavefront each): every workgroup first gathers the
+// whole right-hand side into LDS, so fewer, larger workgroups repeat that less
+constexpr int kDenseWaves = 16;
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_dense_solve(const T *Ainv, int n, const long *rows, const double *vol,
-                                                     const uint8_t *flag, int singular, int pinrow, const T *b,
-                                                     T *x) {
+__global__ void __launch_bounds__(64 * kDenseWaves) k_dense_solve(const T *Ainv, int n, const long *rows,
+                                                                   const double *vol, const uint8_t *flag,
+                                                                   int singular, int pinrow, const T *b, T *x) {
     __shared__ double rhs[kDenseMaxRows];
-    __shared__ double red[2][4];
+    __shared__ double red[2][kDenseWaves];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     double bmean = 0;
     if (singular) {
         double sv = 0, sb = 0;
-        for (int c = t; c < n; c += 256) {
+        for (int c = t; c < n; c += 64 * kDenseWaves) {
             const long q = rows[c];
             if (flag[q] != F_FLUID) continue;
             sv += vol[c];
@@ -1753,11 +1757,11 @@ __global__ void __launch_bounds__(256) k_dense_solve(const T *Ainv, int n, const
         for (int o = 32; o > 0; o >>= 1) { sv += __shfl_down(sv, o, 64); sb += __shfl_down(sb, o, 64); }
         if (lane == 0) { red[0][wave] = sv; red[1][wave] = sb; }
         __syncthreads();
-        const double V = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-        const double B = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        double V = 0, B = 0;
+        for (int w = 0; w < kDenseWaves; w++) { V += red[0][w]; B += red[1][w]; }
         bmean = V > 0 ? B / V : 0;
     }
-    for (int c = t; c < n; c += 256) {
+    for (int c = t; c < n; c += 64 * kDenseWaves) {
         const long q = rows[c];
         double v = (double)b[q];
         if (singular) {
@@ -1767,7 +1771,7 @@ __global__ void __launch_bounds__(256) k_dense_solve(const T *Ainv, int n, const
         rhs[c] = v;
     }
     __syncthreads();
-    const int r = blockIdx.x * 4 + wave;
+    const int r = blockIdx.x * kDenseWaves + wave;
     if (r >= n) return;
     const T *row = Ainv + (long)r * n;
     double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
@@ -1788,8 +1792,8 @@ template <typename T>
 void launch_dense_solve(hipStream_t st, const T *Ainv, int n, const long *rows, const double *vol,
                         const uint8_t *flag, int singular, int pinrow, const T *b, T *x) {
     if (n > kDenseMaxRows) abort();
-    hipLaunchKernelGGL((k_dense_solve<T>), dim3((n + 3) / 4), dim3(256), 0, st, Ainv, n, rows, vol, flag, singular,
-                       pinrow, b, x);
+    hipLaunchKernelGGL((k_dense_solve<T>), dim3((n + kDenseWaves - 1) / kDenseWaves), dim3(64 * kDenseWaves), 0, st,
+                       Ainv, n, rows, vol, flag, singular, pinrow, b, x);
 }
 
 // explicit instantiations
