@@ -140,6 +140,28 @@ def test_cavity_mixed_precision(new_gpu, new_oracle, cases, I, mode):
     assert (g.last_norms() <= 1e-10).all()
 
 
+def test_cavity_128_mixed_vs_fp64(new_gpu, cases, I):
+    """The lid-driven cavity at 128^3 (a pinned, singular pressure system:
+    the case the mixed path's periodic restarts of BiCGSTAB are for) for 50
+    steps at tol 1e-10 in both precisions: the mixed path within 1e-6 rel-L2
+    of FP64 in u and p, every solve converged, max|div U| <= 1e-8."""
+    res = {}
+    for prec in (I.PREC_FP64, I.PREC_MIXED_KRYLOV):
+        g = new_gpu()
+        cases.setup_cavity(g, n=128, Re=1000.0, dt=0.002, tol=1e-10, precision=prec)
+        g.iterate(50)
+        assert (g.last_norms() <= 1e-10).all(), (prec, g.last_norms())
+        assert g.max_div() <= 1e-8
+        res[prec] = (np.stack([g.get(n)[INNER] for n in ("U1", "U2", "U3")]), g.get("P")[INNER],
+                     g.last_iters().tolist())
+        g.close()
+    eu = rel_l2(res[I.PREC_MIXED_KRYLOV][0], res[I.PREC_FP64][0])
+    ep = rel_l2(res[I.PREC_MIXED_KRYLOV][1], res[I.PREC_FP64][1])
+    print(f"cavity 128^3, 50 steps: mixed vs FP64 rel-L2 u {eu:.2e} p {ep:.2e}, its "
+          f"{res[I.PREC_MIXED_KRYLOV][2]} / {res[I.PREC_FP64][2]}", flush=True)
+    assert eu <= 1e-6 and ep <= 1e-6
+
+
 def test_sphere_500_steps_mixed_vs_fp64(new_gpu, cases, I):
     """The sphere workload as configured (256x128x128, convective outflow,
     Re 300, dt 0.002) for its full 500 steps at tol 1e-9, in the mixed path
