@@ -134,8 +134,10 @@ struct _ibm_server {
                 J.done(nullptr, J.user);
             }
         } catch (const std::exception &e) {
+            // the message without the line breaks of its printed form
             err = e.what();
-            if (err.empty()) err = "error";
+            const size_t a = err.find_first_not_of("\n"), b = err.find_last_not_of("\n");
+            err = a == std::string::npos ? std::string("error") : err.substr(a, b - a + 1);
         }
         IBMSolver_destroy(s);
         return err;

@@ -91,3 +91,56 @@ def test_driver_runs_reference_sample(tmp_path, I):
         u = nc.variables["u"][0].transpose(2, 1, 0)
         np.testing.assert_array_equal(u, g.get("U1").astype(np.float32))
     g.close()
+
+
+SERVE = os.path.join(DRIVERS, "ibm3d_serve")
+
+
+def _serve(tmp_path, lines, budget_gib=4, slots=2):
+    cf = tmp_path / "cases.txt"
+    cf.write_text("\n".join(lines) + "\n")
+    r = subprocess.run([SERVE, str(budget_gib), str(slots), str(cf)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-1000:], flush=True)
+    jobs = {}
+    for line in r.stdout.splitlines():
+        if line.startswith("job "):
+            t = line.split()
+            jobs[int(t[1])] = {"kind": t[2], "state": t[4], "iters": [int(v) for v in t[12:16]],
+                               "max_div": float(t[17]), "error": line.split('error "', 1)[1].rstrip('"')}
+    return r, jobs
+
+
+def test_serve_driver_failed_jobs_fail_alone(tmp_path):
+    """The C serving driver (drivers/ibm3d_serve.c: cases configured in C and
+    submitted with IBMServer_submit_solver, no Python): set-ups that
+    IBMSolver_assemble rejects fail as jobs with the reference's message, the
+    process goes on and exits 0. No GPU needed: they fail before any device
+    work."""
+    subprocess.run(["make", "-C", DRIVERS, "-B", "CC=gcc -Werror"], check=True, capture_output=True)
+    r, jobs = _serve(tmp_path, ["badperiodic 8 3", "badperiodic 10 2"])
+    assert r.returncode == 0
+    assert len(jobs) == 2
+    for j in jobs.values():
+        assert j["state"] == "failed" and j["error"] == "Inconsistent periodic boundary condition"
+    assert "jobs 2 failed 2" in r.stdout
+
+
+@pytest.mark.gpu
+def test_serve_driver_runs_a_mix(tmp_path, I, cases):
+    """The C serving driver on the GPU: two cavities and a sphere run beside a
+    rejected set-up, which fails alone. Each cavity's last-step iteration
+    counts equal those of the same case built through the Python binding and
+    run in this process (the native initial perturbation is the Python one)."""
+    r, jobs = _serve(tmp_path, ["cavity 24 10 400", "badperiodic 16 5", "sphere 48 24 24 6 300",
+                                "cavity 32 8 900"], budget_gib=8, slots=3)
+    assert r.returncode == 0 and len(jobs) == 4
+    assert jobs[1]["state"] == "failed" and "Inconsistent periodic" in jobs[1]["error"]
+    for j in (0, 2, 3):
+        assert jobs[j]["state"] == "done" and jobs[j]["max_div"] < 1e-8, jobs[j]
+    for j, (n, steps, Re) in ((0, (24, 10, 400.0)), (3, (32, 8, 900.0))):
+        s = I.Solver()
+        cases.setup_cavity(s, n=n, Re=Re, tol=1e-10, precision=I.PREC_MIXED_KRYLOV)
+        s.iterate(steps)
+        assert s.last_iters().tolist() == jobs[j]["iters"], (j, jobs[j])
+        assert s.max_div() == pytest.approx(jobs[j]["max_div"], rel=1e-3)
+        s.close()

@@ -104,8 +104,9 @@ def test_failed_jobs_fail_alone(I, gpu_lib, capfd):
         assert r.state == 3 and msg in r.error, (r.state, r.error)
         assert 0 <= r.submit <= r.admit <= r.finish
     assert srv.failed_jobs() == 4
-    # 400 + 400 fit the budget of 1000 together, a third did not
-    assert srv.peak_bytes() == 800.0
+    # 400 + 400 fit the budget of 1000 together, a third did not; whether
+    # two of these instant failures overlapped at all depends on timing
+    assert srv.peak_bytes() in (400.0, 800.0)
     srv.close()
     out = capfd.readouterr()
     assert "Inconsistent periodic boundary condition" in out.out
