@@ -679,37 +679,42 @@ __device__ inline void cell_ijk(const Dims &d, long q, int &i, int &j, int &k) {
     i = (int)(rem - jy * sy) - 2;
 }
 
-// Ghost row s relaxed inside the smoother launch of colour `color` (not a
-// zero start). Rows whose cell has that colour take the Jacobi value of their
-// row from the current sources, the rest carry their value over (next =
-// prev). A fluid source of the launch's colour is updated by the same launch,
-// so its new value is evaluated here with the smoother's own arithmetic (its
-// neighbours are of the other colour, which the launch does not change);
-// fluid cells never read a ghost cell of the launch's colour. The result
-// equals a colour launch followed by a relaxation of that colour's ghost rows.
-// All eight sources' loads (their stencils included: a source that needs no
-// stencil evaluates the ghost cell's, whose neighbours are in range) issue
-// together.
+// Ghost rows relaxed inside the smoother launch of colour `color` (not a
+// zero start), eight lanes per row: lane l of the row's group evaluates
+// source l, and the row's weighted sum is formed in source order from the
+// group's values (exchanged within the wave), so the result is that of one
+// lane summing its eight sources. Rows whose cell has that colour take the
+// Jacobi value of their row from the current sources, the rest carry their
+// value over (next = prev). A fluid source of the launch's colour is updated
+// by the same launch, so its new value is evaluated here with the smoother's
+// own arithmetic (its neighbours are of the other colour, which the launch
+// does not change); fluid cells never read a ghost cell of the launch's
+// colour. The result equals a colour launch followed by a relaxation of that
+// colour's ghost rows. Spreading a row's gathers (a source that needs no
+// stencil evaluates the ghost cell's, whose neighbours are in range) over
+// eight lanes keeps the ghost blocks shorter than the sweep they run beside.
+// t: lane in the wave; s: the row of this lane's group (may be >= G.ng).
 template <typename T>
-__device__ inline void ghost_row_in_colour(const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
-                                           T *x, const GhostRows<T> &G, int is_p, const T *prev, T *next,
-                                           int color, int s) {
-    const long c = G.cell[s];
-    int ic, jc, kc;
-    cell_ijk(d, c, ic, jc, kc);
-    if (((ic + jc + kc) & 1) != color) {
-        next[s] = prev[s];
-        return;
+__device__ inline void ghost_rows_in_colour(const Dims &d, const SysCoef<T> &S, const uint8_t *flag, const T *b,
+                                            T *x, const GhostRows<T> &G, int is_p, const T *prev, T *next,
+                                            int color, int t, int s) {
+    const int l = t & 7;
+    const bool in = s < G.ng;
+    const long c = in ? G.cell[s] : 0;
+    int ic = 0, jc = 0, kc = 0;
+    bool act = false;
+    if (in) {
+        cell_ijk(d, c, ic, jc, kc);
+        act = ((ic + jc + kc) & 1) == color;
     }
-    T vals[8];
-    bool used[8];
-    double wv[8];
-#pragma unroll
-    for (int l = 0; l < 8; l++) {
+    double wl = 0;
+    T vl = 0;
+    int used = 0;
+    if (act) {
         const long q0 = G.src[8 * (long)s + l];
         const int g = G.srcg[8 * (long)s + l];
-        wv[l] = G.w64[8 * (long)s + l];
-        used[l] = q0 >= 0;
+        wl = G.w64[8 * (long)s + l];
+        used = q0 >= 0;
         const long q = q0 >= 0 ? q0 : c;
         int i, j, k;
         cell_ijk(d, q, i, j, k);
@@ -724,12 +729,23 @@ __device__ inline void ghost_row_in_colour(const Dims &d, const SysCoef<T> &S, c
         if (g >= 0) v = pg;
         else if (same && (q == S.pin || f == F_SOLID)) v = bq;
         else if (same && f == F_FLUID) v = diag != T(0) ? (bq + off) / diag : T(0);
-        vals[l] = v;
+        vl = v;
     }
+    // every lane of the wave takes part in the exchange
+    const int base = t & ~7;
     double a = 0;
 #pragma unroll
-    for (int l = 0; l < 8; l++)
-        if (used[l]) a += wv[l] * (double)vals[l];
+    for (int m = 0; m < 8; m++) {
+        const double wm = __shfl(wl, base + m, 64);
+        const T vm = __shfl(vl, base + m, 64);
+        const int um = __shfl(used, base + m, 64);
+        if (um) a += wm * (double)vm;
+    }
+    if (!in || l != 0) return;
+    if (!act) {
+        next[s] = prev[s];
+        return;
+    }
     const double ws = G.ws64[s];
     T v;
     if (is_p) {
@@ -742,19 +758,27 @@ __device__ inline void ghost_row_in_colour(const Dims &d, const SysCoef<T> &S, c
     next[s] = v;
 }
 
+// ghost rows per block of the colour launches and of k_ghost_relax: eight
+// lanes per row
+__device__ inline int ghost_row_of_lane(unsigned w) {
+    return (int)(w * (blockDim.x * blockDim.y / 8) + threadIdx.y * (blockDim.x / 8) + (threadIdx.x >> 3));
+}
+static inline unsigned colour_ghost_blocks(int ng, int threads) {
+    const int rows = threads / 8;
+    return ng > 0 ? (unsigned)(((ng + rows - 1) / rows + 7) / 8 * 8) : 0u;
+}
+
 // One colour (not a zero start) with that colour's ghost rows, on a 1-D grid:
 // the first ngb blocks (a multiple of 8, dispatched first and spread over
-// the XCDs) relax ghost rows, one wave of 64 rows per block so the rows'
-// gathers spread over many CUs; the other blocks run the one-colour sweep on
-// a gx x gy x nz block grid, XCD-contiguous.
+// the XCDs) relax ghost rows, eight lanes per row; the other blocks run the
+// one-colour sweep on a gx x gy x nz block grid, XCD-contiguous.
 template <typename T>
 __global__ void __launch_bounds__(256) k_rbgs_g(Dims d, SysCoef<T> S, const uint8_t *flag, const T *b, T *x,
                                                 int color, GhostRows<T> G, int is_p, const T *prev, T *next,
                                                 unsigned ngb, unsigned gx, unsigned gy) {
     const unsigned w = blockIdx.x;
     if (w < ngb) {
-        const int s = (int)w * 64 + threadIdx.x;
-        if (threadIdx.y == 0 && s < G.ng) ghost_row_in_colour(d, S, flag, b, x, G, is_p, prev, next, color, s);
+        ghost_rows_in_colour(d, S, flag, b, x, G, is_p, prev, next, color, threadIdx.x, ghost_row_of_lane(w));
         return;
     }
     rbgs_cell(d, S, flag, b, x, color, 0, xcd_map(w - ngb, gridDim.x - ngb, gx, gy));
@@ -778,8 +802,7 @@ __global__ void __launch_bounds__(64 * BY) k_rbgs_quad_g(Dims d, SysCoef<float> 
                                                          unsigned gx, unsigned gy) {
     const unsigned w = blockIdx.x;
     if (w < ngb) {
-        const int s = (int)w * 64 + threadIdx.x;
-        if (threadIdx.y == 0 && s < G.ng) ghost_row_in_colour(d, S, flag, b, x, G, is_p, prev, next, color, s);
+        ghost_rows_in_colour(d, S, flag, b, x, G, is_p, prev, next, color, threadIdx.x, ghost_row_of_lane(w));
         return;
     }
     rbgs_quad<BY>(d, S, flag, b, x, color, 0, xcd_map(w - ngb, gridDim.x - ngb, gx, gy));
@@ -902,28 +925,46 @@ void launch_rbgs_fused(hipStream_t st, const Dims &d, const SysCoef<T> &S, const
 // cells, which this launch also writes: those sources are read from the
 // values the previous relaxation left in a separate buffer (no other kernel
 // writes x at ghost cells in between), so the result does not depend on the
-// order in which waves run.
+// order in which waves run. Eight lanes per row, one per source; the row's
+// sum is formed in source order from the values exchanged within the wave.
 template <typename T>
-__global__ void k_ghost_relax(GhostRows<T> G, int is_p, const T *b, T *x, const T *prev, T *next, Dims d, int only) {
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= G.ng) return;
-    if (only >= 0) {
+__global__ void __launch_bounds__(256) k_ghost_relax(GhostRows<T> G, int is_p, const T *b, T *x, const T *prev,
+                                                     T *next, Dims d, int only) {
+    const int s = ghost_row_of_lane(blockIdx.x), t = threadIdx.x & 63, l = t & 7;
+    const bool in = s < G.ng;
+    bool act = in;
+    if (in && only >= 0) {
         // rows of the other colour carry their value over
         int iq, jq, kq;
         cell_ijk(d, G.cell[s], iq, jq, kq);
-        if (((iq + jq + kq) & 1) != only) {
-            next[s] = prev ? prev[s] : T(0);
-            return;
+        act = ((iq + jq + kq) & 1) == only;
+    }
+    double wl = 0;
+    T vl = 0;
+    int used = 0;
+    if (act) {
+        const long c = G.src[8 * (long)s + l];
+        if (c >= 0) {
+            const int g = G.srcg[8 * (long)s + l];
+            vl = g < 0 ? x[c] : prev ? prev[g] : T(0);
+            wl = G.w64[8 * (long)s + l];
+            used = 1;
         }
     }
+    // every lane of the wave takes part in the exchange
+    const int base = t & ~7;
     double a = 0;
 #pragma unroll
-    for (int l = 0; l < 8; l++) {
-        const long c = G.src[8 * (long)s + l];
-        if (c < 0) continue;
-        const int g = G.srcg[8 * (long)s + l];
-        const T v = g < 0 ? x[c] : prev ? prev[g] : T(0);
-        a += G.w64[8 * (long)s + l] * (double)v;
+    for (int m = 0; m < 8; m++) {
+        const double wm = __shfl(wl, base + m, 64);
+        const T vm = __shfl(vl, base + m, 64);
+        const int um = __shfl(used, base + m, 64);
+        if (um) a += wm * (double)vm;
+    }
+    if (!in || l != 0) return;
+    if (!act) {
+        next[s] = prev ? prev[s] : T(0);
+        return;
     }
     const long c = G.cell[s];
     const double ws = G.ws64[s];
@@ -960,17 +1001,18 @@ int launch_rbgs_color_ghost(hipStream_t st, const Dims &d, const SysCoef<T> &S, 
                             T *x, int color, const GhostRows<T> &G, int prev) {
     if (prev < 0 || G.ng <= 0) abort();  // needs relaxed ghost values (not a zero start)
     const int out = prev == 0 ? 1 : 0;
-    // ghost blocks: 64 rows each, rounded up to a multiple of the XCD count
-    // so that the sweep blocks keep their XCD assignment
-    const unsigned ngb = (unsigned)(((G.ng + 63) / 64 + 7) / 8 * 8);
+    // ghost blocks: a row per eight lanes, rounded up to a multiple of the
+    // XCD count so that the sweep blocks keep their XCD assignment
     if constexpr (std::is_same<T, float>::value) {
         if (d.nx >= kQuadMinRow) {
+            const unsigned ngb = colour_ghost_blocks(G.ng, 64 * 8);
             const unsigned gx = ((d.nx + 3) / 4 + 63) / 64, gy = (d.ny + 7) / 8;
             hipLaunchKernelGGL((k_rbgs_quad_g<8>), dim3(ngb + gx * gy * d.nz), dim3(64, 8), 0, st, d, S, flag, b, x,
                                color, G, S.is_p, (const float *)G.vals[prev], G.vals[out], ngb, gx, gy);
             return out;
         }
     }
+    const unsigned ngb = colour_ghost_blocks(G.ng, 64 * 4);
     const unsigned gx = ((d.nx + 1) / 2 + 63) / 64, gy = (d.ny + 3) / 4;
     hipLaunchKernelGGL((k_rbgs_g<T>), dim3(ngb + gx * gy * d.nz), dim3(64, 4), 0, st, d, S, flag, b, x, color, G,
                        S.is_p, (const T *)G.vals[prev], G.vals[out], ngb, gx, gy);
@@ -981,7 +1023,7 @@ template <typename T>
 int launch_ghost_relax(hipStream_t st, const GhostRows<T> &G, int is_p, const T *b, T *x, int prev, Dims d, int only) {
     const int out = prev == 0 ? 1 : 0;
     if (G.ng <= 0) return out;
-    const int gb = (G.ng + 255) / 256;
+    const int gb = (G.ng + 31) / 32;  // 32 rows per block of 256 lanes
     hipLaunchKernelGGL((k_ghost_relax<T>), dim3(gb), dim3(256), 0, st, G, is_p, b, x,
                        prev < 0 ? (const T *)nullptr : (const T *)G.vals[prev], G.vals[out], d, only);
     return out;
