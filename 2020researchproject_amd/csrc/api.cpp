@@ -237,6 +237,12 @@ void IBMSolver_set_mg_params(IBMSolver *s, double gamma, int pre, int post, int 
     s->mg_smoother = smoother;
 }
 
+void IBMSolver_set_mg_relaxation(IBMSolver *s, double omega) {
+    if (s->assembled) fatal("IBMSolver_set_mg_relaxation must precede IBMSolver_assemble");
+    if (!(omega > 0 && omega < 2)) fatal("multigrid over-relaxation factor must lie in (0, 2)");
+    s->mg_omega = omega;
+}
+
 void IBMSolver_set_mg_light_levels(IBMSolver *s, int from_level) {
     if (s->assembled) fatal("IBMSolver_set_mg_light_levels must precede IBMSolver_assemble");
     s->mg_light_from = from_level;

@@ -99,6 +99,9 @@ struct SysCoef {
     int skip;       // drop faces to F_SOLID interior neighbours (coarse levels)
     const uint8_t *fmask;  // skip: per cell, bit 2a / 2a+1 set when the minus / plus
                            // neighbour along axis a is an interior F_SOLID cell
+    T omega = T(1);        // red-black smoother: over-relaxation factor of the
+                           // fluid rows (1: Gauss-Seidel); read by the smoother
+                           // kernels only
 };
 
 }  // namespace ibm

@@ -87,7 +87,8 @@ __device__ inline void rbgs_quad(const Dims &d, const SysCoef<float> &S, const u
         } else if (f != F_FLUID) {
             put[h] = inside && zero_in;
         } else {
-            res[h] = diag != 0.f ? (bv + off) / diag : 0.f;
+            // over-relaxed from the cell's old value (0 from a zero start)
+            res[h] = sor_value(S.omega, quad_pick(xc, h, p), diag != 0.f ? (bv + off) / diag : 0.f);
             put[h] = inside;
         }
     }

@@ -223,6 +223,7 @@ struct _ibm_solver {
     int mg_pre = 2, mg_post = 2; // smoothing sweeps per level
     int mg_smoother = 1;         // 0 fused red-black sweeps, 1 one launch per colour
     int mg_light_from = 1;       // levels >= this (if > 0) smooth with one sweep before and after
+    double mg_omega = 1.3;       // over-relaxation factor of the pressure multigrid smoother
     int mom_solver = 0;          // IBMSolver_set_momentum_solver: 0 automatic, 1 BiCGSTAB
     int mom_sweeps = -1;         // FP32 sweeps per refinement round of the momentum solves (0: not used; -1: unset)
     double mom_rate = 0;         // estimated red-black sweep contraction of the momentum operators

@@ -204,7 +204,7 @@ __device__ inline T first_colour_of(const SysCoef<T> &S, unsigned f, int i, int 
     if ((i + j + k) & 1) return T(0);
     if (q == S.pin || f == F_SOLID) return bv;
     if (f != F_FLUID) return T(0);
-    return diag != T(0) ? (bv + T(0)) / diag : T(0);
+    return sor_value(S.omega, T(0), diag != T(0) ? (bv + T(0)) / diag : T(0));
 }
 template <typename T>
 __device__ inline T first_colour(const SysCoef<T> &S, unsigned f, int i, int j, int k, long q, T bv) {
@@ -629,7 +629,8 @@ void launch_bicg_jacobi_f32(hipStream_t st, const Dims &d, const SysCoef<double>
 }
 
 // ---------------------------------------------------------------------------
-// Red-black Gauss-Seidel on fluid rows, Jacobi on ghost rows after each colour.
+// Red-black Gauss-Seidel (over-relaxed by SysCoef::omega) on fluid rows,
+// Jacobi on ghost rows after each colour.
 // Thread t of a row handles x = 2t + parity so every lane owns one cell.
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -647,6 +648,7 @@ __device__ inline void rbgs_cell(const Dims &d, const SysCoef<T> &S, const uint8
     }
     const uint8_t f = flag[q];
     const T bv = b[q];
+    const T xo = zero_in ? T(0) : x[q];
     T diag, off;
     if (zero_in) {
         diag = stencil_diag(S, i, j, k, q);
@@ -659,7 +661,7 @@ __device__ inline void rbgs_cell(const Dims &d, const SysCoef<T> &S, const uint8
         if (zero_in) x[q] = T(0);
         return;
     }
-    x[q] = diag != T(0) ? (bv + off) / diag : T(0);
+    x[q] = sor_value(S.omega, xo, diag != T(0) ? (bv + off) / diag : T(0));
 }
 
 template <typename T>
@@ -686,11 +688,15 @@ __device__ inline void cell_ijk(const Dims &d, long q, int &i, int &j, int &k) {
 // lane summing its eight sources. Rows whose cell has that colour take the
 // Jacobi value of their row from the current sources, the rest carry their
 // value over (next = prev). A fluid source of the launch's colour is updated
-// by the same launch, so its new value is evaluated here with the smoother's
-// own arithmetic (its neighbours are of the other colour, which the launch
-// does not change); fluid cells never read a ghost cell of the launch's
-// colour. The result equals a colour launch followed by a relaxation of that
-// colour's ghost rows. Spreading a row's gathers (a source that needs no
+// by the same launch, so its new value is evaluated here with the
+// Gauss-Seidel formula of the smoother (its neighbours are of the other
+// colour, which the launch does not change); fluid cells never read a ghost
+// cell of the launch's colour. With SysCoef::omega = 1 the result equals a
+// colour launch followed by a relaxation of that colour's ghost rows; an
+// over-relaxed launch writes xo + omega (gs - xo) at such a source while the
+// ghost row takes its Gauss-Seidel value gs (the old value xo is being
+// overwritten by the same launch and cannot be read), a fixed linear map all
+// the same. Spreading a row's gathers (a source that needs no
 // stencil evaluates the ghost cell's, whose neighbours are in range) over
 // eight lanes keeps the ghost blocks shorter than the sweep they run beside.
 // t: lane in the wave; s: the row of this lane's group (may be >= G.ng).

@@ -23,4 +23,12 @@ __device__ inline void axis_term(const AxisCoef<T> &A, int ii, bool sm, bool sp,
     axis_term4(c.x, c.y, c.z, c.w, sm, sp, xm, xp, diag, off);
 }
 
+// New value of a fluid row from its old value xo and its Gauss-Seidel value
+// gs: successive over-relaxation xo + om (gs - xo), and gs itself (bit for
+// bit) for om = 1.
+template <typename T>
+__device__ inline T sor_value(T om, T xo, T gs) {
+    return om == T(1) ? gs : xo + om * (gs - xo);
+}
+
 }  // namespace ibm

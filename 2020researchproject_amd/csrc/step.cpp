@@ -356,6 +356,10 @@ void build_mg(IBMSolver *s) {
         pk.insert(pk.end(), s->dy.begin(), s->dy.end());
         pk.insert(pk.end(), s->dz.begin(), s->dz.end());
         L0.dxyz.upload(pk);
+        // the smoother's over-relaxation factor (the operator kernels do not
+        // read it)
+        s->sys[3].S64.omega = s->mg_omega;
+        s->sys[3].S32.omega = (float)s->mg_omega;
     }
     std::vector<double> xf = s->xf, yf = s->yf, zf = s->zf;
     const int lo[3] = {3, 2, 4}, hi[3] = {1, 0, 5};
@@ -440,6 +444,8 @@ void build_mg(IBMSolver *s) {
             for (int h = 0; h < 2; h++) bct[a][h] = s->bc[face_bc(a, h)].type;
         build_system(s, D, C, Km, Kp, m, bct, 3, L.sys);
         L.sys.S64.skip = L.sys.S32.skip = 1;
+        L.sys.S64.omega = s->mg_omega;
+        L.sys.S32.omega = (float)s->mg_omega;
         // faces to inactive (solid) neighbours, per cell
         {
             std::vector<uint8_t> fm(L.d.ncell, 0);

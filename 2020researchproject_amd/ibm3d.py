@@ -105,6 +105,7 @@ _OPTIONAL = {
     "IBMSolver_get_device_bytes": (C.c_double, [_vp]),
     "IBMSolver_get_mg_levels": (C.c_int, [_vp, _ip, C.c_int]),
     "IBMSolver_set_mg_light_levels": (None, [_vp, C.c_int]),
+    "IBMSolver_set_mg_relaxation": (None, [_vp, C.c_double]),
     "IBMSolver_estimate_device_bytes": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "IBMServer_new": (_vp, [C.c_double, C.c_int]),
     "IBMServer_submit": (C.c_int, [_vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, _vp]),
@@ -263,6 +264,10 @@ class Solver:
 
     def set_mg_params(self, gamma: float = 3.5, pre: int = 2, post: int = 2, smoother: int = 1):
         self.lib.IBMSolver_set_mg_params(self.h, gamma, pre, post, smoother)
+
+    def set_mg_relaxation(self, omega: float = 1.3):
+        """Over-relaxation factor of the pressure multigrid smoother."""
+        self.lib.IBMSolver_set_mg_relaxation(self.h, float(omega))
 
     def set_mg_light_levels(self, from_level: int = 1):
         if hasattr(self.lib, "IBMSolver_set_mg_light_levels"):

@@ -9,7 +9,8 @@
 // AddressSanitizer, every load of the kernel (including the 16-byte loads
 // that run past the last cell of a row) is checked against the allocation.
 // Two launches (colour 0, then colour 1) are compared bit for bit with a
-// direct sweep that applies the stencil cell by cell, as k_rbgs does, and
+// direct sweep that applies the stencil cell by cell, as k_rbgs does
+// (Gauss-Seidel, and over-relaxed), and
 // the four-cells-per-lane residual with the one-cell residual at every fluid
 // cell (and with 0 at the others).
 //
@@ -74,7 +75,10 @@ static void run_direct(const Dims &d, const SysCoef<float> &S, const uint8_t *fl
                     const bool sp = S.skip && (ii < n[a] - 1 || A.periodic) && flag[qp] == F_SOLID;
                     axis_term(A, ii, sm, sp, x[qm], x[qp], diag, off);
                 }
-                x[q] = diag != 0.f ? (b[q] + off) / diag : 0.f;
+                const float gs = diag != 0.f ? (b[q] + off) / diag : 0.f;
+                // over-relaxation: xo + om (gs - xo), gs itself for om = 1
+                const float om = S.omega;
+                x[q] = om == 1.f ? gs : x[q] + om * (gs - x[q]);
             }
 }
 
@@ -113,7 +117,8 @@ static float direct_resid(const Dims &d, const SysCoef<float> &S, const uint8_t 
     return b[q] - (diag * x[q] - off);
 }
 
-static bool one_case(std::mt19937_64 &rng, int nx, int ny, int nz, int per, int skip, int pin, int zero_in) {
+static bool one_case(std::mt19937_64 &rng, int nx, int ny, int nz, int per, int skip, int pin, int zero_in,
+                     float omega) {
     const Dims d = make_dims(nx, ny, nz);
     std::uniform_real_distribution<double> U(0.1, 1.0), V(-1.0, 1.0);
     std::vector<std::vector<float>> coef(12), packed(3);
@@ -158,6 +163,7 @@ static bool one_case(std::mt19937_64 &rng, int nx, int ny, int nz, int per, int 
                 }
                 fmask[cidx(d, i, j, k)] = (uint8_t)bits;
             }
+    S.omega = omega;
     S.skip = skip;
     S.fmask = skip ? fmask.data() : nullptr;
     S.pin = pin ? cidx(d, 0, 0, 0) : -1;
@@ -196,8 +202,8 @@ static bool one_case(std::mt19937_64 &rng, int nx, int ny, int nz, int per, int 
                 }
             }
     if (bad)
-        fprintf(stderr, "case %dx%dx%d per=%d skip=%d pin=%d zero=%d: %ld mismatches\n", nx, ny, nz, per, skip, pin,
-                zero_in, bad);
+        fprintf(stderr, "case %dx%dx%d per=%d skip=%d pin=%d zero=%d omega=%g: %ld mismatches\n", nx, ny, nz, per,
+                skip, pin, zero_in, (double)omega, bad);
     return bad == 0;
 }
 
@@ -214,11 +220,12 @@ int main() {
     bool ok = true;
     for (const C &c : cs)
         for (int skip = 0; skip < 2; skip++)
-            for (int zero = 0; zero < 2; zero++) {
-                const int pin = (c.nx + skip + zero) & 1;
-                ok &= one_case(rng, c.nx, c.ny, c.nz, c.per, skip, pin, zero);
-                ncase++;
-            }
+            for (int zero = 0; zero < 2; zero++)
+                for (float omega : {1.0f, 1.3f}) {
+                    const int pin = (c.nx + skip + zero) & 1;
+                    ok &= one_case(rng, c.nx, c.ny, c.nz, c.per, skip, pin, zero, omega);
+                    ncase++;
+                }
     if (!ok) return 1;
     printf("ok %d\n", ncase);
     return 0;

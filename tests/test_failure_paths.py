@@ -89,6 +89,28 @@ def test_odd_periodic_axis_with_multigrid_is_refused():
     assert "periodic x axis with an odd cell count (9)" in r.stderr
 
 
+@pytest.mark.parametrize("omega", [0.0, 2.0, -0.5, float("nan")])
+def test_multigrid_over_relaxation_outside_0_2_is_refused(omega):
+    """IBMSolver_set_mg_relaxation accepts 0 < omega < 2 (the range where
+    successive over-relaxation of a positive-definite row set converges) and
+    prints and aborts otherwise; 1.3 is accepted."""
+    r = run_case(HIP_LIB, f"""
+        s.set_mg_relaxation({omega!r})
+        print("accepted")
+    """.replace("nan", "float('nan')"))
+    assert aborted(r), (r.returncode, r.stdout, r.stderr)
+    assert "over-relaxation factor must lie in (0, 2)" in r.stderr
+    assert "accepted" not in r.stdout
+    r = run_case(HIP_LIB, """
+        s.set_mg_relaxation(1.3)
+        s.set_bc(63, I.BC_STATIONARY_WALL)
+        s.set_obstacle(None)
+        s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, 1e-10)
+        print("accepted")
+    """)
+    assert r.returncode == 0 and "accepted" in r.stdout, (r.returncode, r.stderr)
+
+
 def test_mixed_defect_correction_refuses_other_pressure_solvers():
     r = run_case(HIP_LIB, """
         s.set_bc(63, I.BC_STATIONARY_WALL)
