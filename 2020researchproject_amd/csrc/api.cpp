@@ -704,13 +704,20 @@ extern "C" void IBMSolver_assemble(IBMSolver *s) {
                 case_error(buf);
             }
     }
-    DevBytesScope account(&s->dev_bytes);
-    init_device(s);
     s->p_singular = true;  // <- src/ibm3d_setup.c:1598-1603
     for (int i = 0; i < 6; i++)
         if (s->bc[i].type == BC_PRESSURE || s->bc[i].type == BC_VELOCITY_PERIODIC
             || s->bc[i].type == BC_CONVECTIVE_OUTFLOW)
             s->p_singular = false;
+    // The inner FP32 solves of IBM3D_PREC_MIXED stall on a pinned (otherwise
+    // singular) pressure system, whose near-null mode they cannot resolve in
+    // FP32; IBM3D_PREC_MIXED_KRYLOV restarts its FP64 iteration instead.
+    if (s->precision == IBM3D_PREC_MIXED && s->p_singular)
+        fatal("IBM3D_PREC_MIXED needs a pressure-type boundary (BC_PRESSURE, BC_VELOCITY_PERIODIC or "
+              "BC_CONVECTIVE_OUTFLOW): without one the pressure system is pinned and its FP32 inner solves "
+              "stall (use IBM3D_PREC_MIXED_KRYLOV or IBM3D_PREC_FP64)");
+    DevBytesScope account(&s->dev_bytes);
+    init_device(s);
 
     calc_geometry(s);
     calc_lvset_flag(s);

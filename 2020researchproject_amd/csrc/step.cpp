@@ -772,6 +772,15 @@ static void mg_vcycle_f32(IBMSolver *s, bool first_done, float *out) {
 // bnorm > 0: stopping norm of the reference system's right-hand side;
 // bnorm <= 0: use ||b|| of this system (inner solves).
 // ---------------------------------------------------------------------------
+// Stagnation test of the restarted (mixed-precision) BiCGSTAB: from iteration
+// kStallFrom after a (re)start, a recurrence residual above kStallFactor times
+// the one kStallWindow iterations earlier restarts the iteration (measured on
+// the pinned cavity and channel pressure systems: 40-50 iterations per step
+// without it, 8-20 with it, as FP64; unchanged where the FP32 cycle does not
+// stall, the sphere and urban cases).
+constexpr int kStallWindow = 3, kStallFrom = 4;
+constexpr double kStallFactor = 0.1;
+
 template <typename T>
 static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, const T *b, T *x, int prec,
                     const NormSpec &ns, double tol, int maxit, double bnorm, double *final_rel, const char *tag,
@@ -819,6 +828,9 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
     // operator and the update in FP32, no FP64 copies
     float *e32a = f32mg || f32mom ? s->ref_e32.p : nullptr, *e32b = f32mg || f32mom ? s->ref_e32b.p : nullptr;
     int it = 0;
+    // recurrence residuals since the last (re)start, for the stagnation test
+    std::vector<double> hist{rn};
+    int it_rs = 0;
     for (;;) {
     while (it < maxit && rn > tol * bn) {
         it++;
@@ -902,15 +914,26 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         finalize_read(s, 2, nv, 3);
         rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
         const bool bad = s->ks_h->breakdown || !std::isfinite(rn);
-        if (restart > 0 && it < maxit && (bad || it % restart == 0) && !(rn <= tol * bn)) {
+        hist.push_back(rn);
+        const int since = it - it_rs;
+        // stagnation: from the fourth iteration after a (re)start, less
+        // than a tenfold reduction over the last three iterations
+        const bool stalled = since >= kStallFrom && rn > kStallFactor * hist[since - kStallWindow];
+        if (restart > 0 && it < maxit && (bad || it % restart == 0 || stalled) && !(rn <= tol * bn)) {
             // Restart from the true residual of the current iterate. With a
             // preconditioner that is not exactly the same linear map every
-            // time (an FP32 multigrid cycle), the recurrences drift from the
-            // true residual; restarting resets them, keeping x.
+            // time (an FP32 multigrid cycle), BiCGSTAB's recurrences lose
+            // their bi-orthogonality; on pinned (otherwise singular)
+            // pressure systems it then stalls near 1e-6 of its starting
+            // residual and a few iterations later may carry the iterate
+            // away. Restarting resets the recurrences, keeping x: at the
+            // first stall, and every `restart` iterations in any case.
             TIMED("residual", 3 * cb, launch_residual<T>(s->stream, d, S, G, flag, b, x, r, ns, s->partial.p));
             launch_bicg_restart<T>(s->stream, d, r, rh, p, v);
             finalize_read(s, 3, na, 2);
             rn = std::sqrt(ns.on ? s->ks_h->rrn : s->ks_h->rr);
+            it_rs = it;
+            hist.assign(1, rn);
             if (!std::isfinite(rn)) break;
             continue;
         }
@@ -926,6 +949,8 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         launch_bicg_restart<T>(s->stream, d, r, rh, p, v);
         finalize_read(s, 3, na, 2);
         rn = rt;
+        it_rs = it;
+        hist.assign(1, rn);
         continue;
     }
     *final_rel = rt / bn;
@@ -1115,6 +1140,9 @@ static int mg_solve_p(IBMSolver *s, const double *b, double *x, int prec, const 
 }
 
 // FP64 iterative refinement around FP32 BiCGSTAB inner solves.
+// BiCGSTAB iterations between restarts when the preconditioner runs in FP32
+static const int kMixedRestart = 40;
+
 static int refine_mixed(IBMSolver *s, int sk, const double *b, double *x, int prec, const NormSpec &ns, double tol,
                         int maxit, double bnorm, double *final_rel) {
     const Dims &d = s->d;
@@ -1153,9 +1181,6 @@ static int refine_mixed(IBMSolver *s, int sk, const double *b, double *x, int pr
     *final_rel = rn / bn;
     return total;
 }
-
-// BiCGSTAB iterations between restarts when the preconditioner runs in FP32
-static const int kMixedRestart = 40;
 
 // Records the outcome of the solve of system sk and reacts like the reference
 // (src/ibm3d_fracstep.c:363-369, 673-679): a non-finite residual aborts,

@@ -43,7 +43,9 @@ void IBMSolver_set_obstacle_sphere(IBMSolver *, double cx, double cy, double cz,
                                    double r);
 
 /* Precision mode of the linear solves (default IBM3D_PREC_FP64). Must be
-   called before IBMSolver_assemble. */
+   called before IBMSolver_assemble. IBM3D_PREC_MIXED needs a pressure-type
+   boundary (BC_PRESSURE, BC_VELOCITY_PERIODIC or BC_CONVECTIVE_OUTFLOW) and
+   SOLVER_BiCGSTAB; IBMSolver_assemble refuses it otherwise. */
 void IBMSolver_set_precision(IBMSolver *, int mode);
 
 /* Multigrid parameters of the pressure preconditioner (no counterpart in

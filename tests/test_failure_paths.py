@@ -111,6 +111,23 @@ def test_multigrid_over_relaxation_outside_0_2_is_refused(omega):
     assert r.returncode == 0 and "accepted" in r.stdout, (r.returncode, r.stderr)
 
 
+def test_mixed_defect_correction_refuses_pinned_pressure():
+    """IBM3D_PREC_MIXED (FP64 defect correction around FP32 BiCGSTAB) refuses a
+    case without a pressure-type boundary, whose pinned pressure system its
+    FP32 inner solves cannot resolve; the refusal comes before any device
+    work. The same case assembles in the other modes (not run here: that
+    needs a GPU)."""
+    r = run_case(HIP_LIB, """
+        s.set_bc(63, I.BC_STATIONARY_WALL)
+        s.set_obstacle(None)
+        s.set_linear_solver(I.SOLVER_BiCGSTAB, I.PRECOND_AMG, 1e-10)
+        s.set_precision(I.PREC_MIXED)
+        s.assemble()
+    """)
+    assert aborted(r), (r.returncode, r.stderr)
+    assert "IBM3D_PREC_MIXED needs a pressure-type boundary" in r.stderr
+
+
 def test_mixed_defect_correction_refuses_other_pressure_solvers():
     r = run_case(HIP_LIB, """
         s.set_bc(63, I.BC_STATIONARY_WALL)
