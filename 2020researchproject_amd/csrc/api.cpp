@@ -243,6 +243,11 @@ void IBMSolver_set_mg_relaxation(IBMSolver *s, double omega) {
     s->mg_omega = omega;
 }
 
+int IBMSolver_get_pressure_mg_bits(IBMSolver *s) {
+    if (!s->assembled) return 0;
+    return s->precision == IBM3D_PREC_MIXED_KRYLOV && !s->p_mg64 ? 32 : 64;
+}
+
 void IBMSolver_set_mg_light_levels(IBMSolver *s, int from_level) {
     if (s->assembled) fatal("IBMSolver_set_mg_light_levels must precede IBMSolver_assemble");
     s->mg_light_from = from_level;
@@ -704,6 +709,7 @@ extern "C" void IBMSolver_assemble(IBMSolver *s) {
                 case_error(buf);
             }
     }
+    s->p_mg64 = false;
     s->p_singular = true;  // <- src/ibm3d_setup.c:1598-1603
     for (int i = 0; i < 6; i++)
         if (s->bc[i].type == BC_PRESSURE || s->bc[i].type == BC_VELOCITY_PERIODIC

@@ -216,6 +216,8 @@ struct _ibm_solver {
     int precision = IBM3D_PREC_FP64;
     bool assembled = false;
     bool p_singular = true;
+    bool p_mg64 = false;         // mixed-Krylov mode: the pressure preconditioner fell back to the FP64
+                                 // multigrid cycle after a solve with repeated stalls
     size_t dev_bytes = 0;        // device memory allocated during setup
     double vd_aplus = 26, vd_utau = 1;  // van Driest damping (TURBMODEL_SMAGORINSKY_VD)
     int vd_walls = 0;

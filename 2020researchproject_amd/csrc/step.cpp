@@ -780,11 +780,16 @@ static void mg_vcycle_f32(IBMSolver *s, bool first_done, float *out) {
 // stall, the sphere and urban cases).
 constexpr int kStallWindow = 3, kStallFrom = 4;
 constexpr double kStallFactor = 0.1;
+// A mixed-Krylov pressure solve that stalls this many times switches the
+// solver's pressure preconditioner to the FP64 cycle for the following steps
+// (the channel at tol 1e-11: 14-39 iterations per step with FP32 cycles and
+// restarts, 10 with the FP64 cycle; the cavity stalls at most once a solve).
+constexpr int kStallFallback = 2;
 
 template <typename T>
 static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, const T *b, T *x, int prec,
                     const NormSpec &ns, double tol, int maxit, double bnorm, double *final_rel, const char *tag,
-                    int restart = 0) {
+                    int restart = 0, int *stalls = nullptr) {
     const Dims &d = s->d;
     DBuf<T> *V = kvec<T>(s->kry);
     T *r = V[0].p, *rh = V[1].p, *p = V[2].p, *v = V[3].p, *ph = V[4].p, *sv = V[5].p, *sh = V[6].p, *t = V[7].p;
@@ -920,6 +925,7 @@ static int bicgstab(IBMSolver *s, const SysCoef<T> &S, const GhostRows<T> &G, co
         // than a tenfold reduction over the last three iterations
         const bool stalled = since >= kStallFrom && rn > kStallFactor * hist[since - kStallWindow];
         if (restart > 0 && it < maxit && (bad || it % restart == 0 || stalled) && !(rn <= tol * bn)) {
+            if (stalled && stalls) ++*stalls;
             // Restart from the true residual of the current iterate. With a
             // preconditioner that is not exactly the same linear map every
             // time (an FP32 multigrid cycle), BiCGSTAB's recurrences lose
@@ -1381,8 +1387,14 @@ static void solve_system(IBMSolver *s, int sk, const double *b, double *x, doubl
         it = refine_mixed(s, sk, b, x, prec, ns, s->tol, maxit, bnorm, &rel);
     } else if (s->precision == IBM3D_PREC_MIXED_KRYLOV) {
         const GhostData &GD = sk == 3 ? s->gp : s->gmom;
-        it = bicgstab<double>(s, s->sys[sk].S64, GD.G64, b, x, prec == 2 ? 3 : prec, ns, s->tol, maxit, bnorm, &rel,
-                              sk == 3 ? "p64" : "u64", prec == 2 ? kMixedRestart : 0);
+        // pressure with the multigrid preconditioner: FP32 cycles (prec 3),
+        // or the FP64 cycle (prec 2) once a solve has stalled kStallFallback
+        // times; restarts only with the FP32 cycles
+        const bool mg32 = prec == 2 && !s->p_mg64;
+        int stalls = 0;
+        it = bicgstab<double>(s, s->sys[sk].S64, GD.G64, b, x, mg32 ? 3 : prec, ns, s->tol, maxit, bnorm, &rel,
+                              sk == 3 ? "p64" : "u64", mg32 ? kMixedRestart : 0, &stalls);
+        if (mg32 && stalls >= kStallFallback) s->p_mg64 = true;
     } else {
         const GhostData &GD = sk == 3 ? s->gp : s->gmom;
         it = bicgstab<double>(s, s->sys[sk].S64, GD.G64, b, x, prec, ns, s->tol, maxit, bnorm, &rel,

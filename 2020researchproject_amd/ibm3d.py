@@ -106,6 +106,7 @@ _OPTIONAL = {
     "IBMSolver_get_mg_levels": (C.c_int, [_vp, _ip, C.c_int]),
     "IBMSolver_set_mg_light_levels": (None, [_vp, C.c_int]),
     "IBMSolver_set_mg_relaxation": (None, [_vp, C.c_double]),
+    "IBMSolver_get_pressure_mg_bits": (C.c_int, [_vp]),
     "IBMSolver_estimate_device_bytes": (C.c_double, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "IBMServer_new": (_vp, [C.c_double, C.c_int]),
     "IBMServer_submit": (C.c_int, [_vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, _vp]),
@@ -268,6 +269,10 @@ class Solver:
     def set_mg_relaxation(self, omega: float = 1.3):
         """Over-relaxation factor of the pressure multigrid smoother."""
         self.lib.IBMSolver_set_mg_relaxation(self.h, float(omega))
+
+    def pressure_mg_bits(self) -> int:
+        """32 or 64: the precision of the pressure multigrid cycle in use."""
+        return int(self.lib.IBMSolver_get_pressure_mg_bits(self.h))
 
     def set_mg_light_levels(self, from_level: int = 1):
         if hasattr(self.lib, "IBMSolver_set_mg_light_levels"):

@@ -75,6 +75,13 @@ void IBMSolver_set_mg_light_levels(IBMSolver *, int from_level);
    preconditioner changes: the stopping test is the same. Before
    IBMSolver_assemble. No counterpart in the reference. */
 void IBMSolver_set_mg_relaxation(IBMSolver *, double omega);
+/* Precision of the pressure multigrid cycle the solver uses now: 32 or 64
+   (0 before IBMSolver_assemble). In IBM3D_PREC_MIXED_KRYLOV it is 32 until a
+   pressure solve stalls twice (a residual falling less than tenfold over
+   three BiCGSTAB iterations, each time followed by a restart from the true
+   residual); the solver then keeps the FP64 cycle for its remaining steps.
+   No counterpart in the reference. */
+int IBMSolver_get_pressure_mg_bits(IBMSolver *);
 
 /* Momentum Helmholtz solver of the IBM3D_PREC_MIXED_KRYLOV mode. 0
    (default): FP32 red-black Gauss-Seidel sweeps inside FP64 iterative

@@ -127,3 +127,11 @@ def test_native_submission_refusals(I, gpu_lib):
     assert gpu_lib.IBMServer_failed_jobs(srv) == 0
     gpu_lib.IBMServer_destroy(srv)
     s.close()  # still the caller's: destroyed here, unassembled
+
+
+def test_pressure_mg_bits_before_assembly(I, gpu_lib):
+    """IBMSolver_get_pressure_mg_bits reports 0 until the solver is assembled
+    (host state only, no GPU needed)."""
+    s = _configured(I, precision=I.PREC_MIXED_KRYLOV)
+    assert s.pressure_mg_bits() == 0
+    s.close()
