@@ -266,6 +266,25 @@ def test_sphere_mixed_precision(new_gpu, new_oracle, cases, I, mode):
     assert g.max_div() < 1e-8
 
 
+@pytest.mark.parametrize("n", [(133, 27, 23), (131, 25, 21)], ids=["nx-mod4-1", "nx-mod4-3"])
+@pytest.mark.parametrize("precision", [0, 1, 2], ids=["fp64", "defect-correction", "fp64-krylov-fp32-mg"])
+def test_sphere_ragged_rows_vs_oracle(new_gpu, new_oracle, cases, precision, n):
+    """Fine rows of at least 128 cells whose length is not a multiple of four
+    (the FP32 four-cells-per-lane sweep and residual kernels take their
+    row-end branches on the fine level) and odd y and z counts, 4 steps
+    against the oracle: flags and ghost sets bit-exact, fields at the bar of
+    each mode."""
+    g, o = new_gpu(), new_oracle()
+    cases.setup_sphere(g, *n, tol=1e-12 if precision == 0 else 1e-10, precision=precision)
+    cases.setup_sphere(o, *n, tol=1e-12)
+    g.iterate(4)
+    o.iterate(4)
+    assert np.array_equal(g.flags(), o.flags())
+    assert np.array_equal(g.ghost_cells(), o.ghost_cells())
+    compare(g, o, 1e-8 if precision == 0 else 1e-6)
+    assert g.max_div() < 1e-8
+
+
 def setup_channel(s, I, cases, model=None, coef=0.0, n=(16, 12, 8), tol=1e-12, **turb):
     import math
     xf = cases.uniform_faces(0.0, 2 * math.pi, n[0])
