@@ -248,6 +248,29 @@ def roofline(stats):
             "launches": dom["count"], "traffic_source": "profiles/traffic.json"}
 
 
+def step_roofline(stats, prof_steps: int, cells: int, step_ms):
+    """The whole step against the HBM roofline: the algorithmic bytes of
+    every timed kernel that states them (each distinct array element once per
+    launch), summed over the instrumented steps and given per cell-step; that
+    sum over those kernels' summed duration ("kernels": every kernel at its
+    own pace) and, per step, over the median time of the timed steps
+    ("step": launch gaps and host waits included)."""
+    if not stats or prof_steps <= 0:
+        return None
+    nbytes = sum(t["bytes"] * t["count"] for t in stats if t["bytes"] > 0)
+    t_bytes = sum(t["ms"] for t in stats if t["bytes"] > 0) * 1e-3
+    t_all = sum(t["ms"] for t in stats) * 1e-3
+    per_step = nbytes / prof_steps
+    wall = float(np.median(step_ms)) * 1e-3
+    if t_bytes <= 0 or wall <= 0:
+        return None
+    kern, step = nbytes / t_bytes / 1e9, per_step / wall / 1e9
+    return {"bytes_per_cell_step": round(per_step / cells, 1), "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "achieved_kernels": round(kern, 1), "frac_kernels": round(kern / HBM_PEAK_GBS, 4),
+            "achieved_step": round(step, 1), "frac_step": round(step / HBM_PEAK_GBS, 4),
+            "kernel_time_with_bytes": round(t_bytes / t_all, 3)}
+
+
 def top_kernels(stats, n: int = 8):
     """[name, share of timed GPU time, avg us, algorithmic GB/s] of the
     kernels with the largest total time in the instrumented steps."""
@@ -397,10 +420,13 @@ def main() -> int:
     if rf:
         out["roofline"] = rf
     out["top_kernels"] = top_kernels(mixed["stats"])
+    sr = step_roofline(mixed["stats"], args.prof_steps, cells, mixed["step_ms"])
+    if sr:
+        out["step_roofline"] = sr
     if args.stats:
         print_stats("mixed", mixed)
     if not args.no_fp64:
-        f64 = run_gpu(I.PREC_FP64, nx, ny, nz, args.steps, args.warmup, args.tol, args.prof_steps if args.stats else 0,
+        f64 = run_gpu(I.PREC_FP64, nx, ny, nz, args.steps, args.warmup, args.tol, args.prof_steps,
                       not args.no_graphs, args.outflow)
         v64 = cells * args.steps / f64["wall"] / 1e6
         out["fp64_value"] = round(v64, 2)
@@ -410,6 +436,9 @@ def main() -> int:
         out["rel_l2_u_mixed_vs_fp64"] = max(rel_l2(a, b) for a, b in zip(mixed["u"], f64["u"]))
         out["fp64_iters_last_step"] = f64["iters"]
         out["cd_abs_diff_mixed_vs_fp64"] = max(abs(a[0] - b[0]) for a, b in zip(mixed["coef_hist"], f64["coef_hist"]))
+        sr = step_roofline(f64["stats"], args.prof_steps, cells, f64["step_ms"])
+        if sr:
+            out["fp64_step_roofline"] = sr
         if args.stats:
             print_stats("fp64", f64)
     if not args.no_serving:
