@@ -1,4 +1,5 @@
-// serve.cpp - request loop over independent cases on one GPU (C5 serving).
+// serve.cpp - request loop over independent cases on one GPU (a stream of
+// mixed-size cavity and sphere cases arriving while others run).
 //
 // A pool of worker threads, one per slot. A worker takes the first queued job
 // (in arrival order) whose footprint fits the remaining device-memory budget,

@@ -56,10 +56,9 @@ def test_cavity16_fp64_parity(new_gpu, new_oracle, cases):
     assert g.iter == o.iter == 5 and g.time == pytest.approx(o.time, rel=1e-15)
 
 
-def test_cavity32_fp64_parity_config1(new_gpu, new_oracle, cases):
-    """The 32^3 lid-driven cavity of the CPU-runnable configuration as stated:
-    Re 100, dt 0.005, seeded perturbation of amplitude 1e-3, all 200 steps,
-    pressure solved to relative residual 1e-10."""
+def test_cavity32_fp64_parity_200_steps(new_gpu, new_oracle, cases):
+    """The 32^3 lid-driven cavity: Re 100, dt 0.005, seeded perturbation of
+    amplitude 1e-3, all 200 steps, solves to relative residual 1e-10."""
     g, o = run_pair(new_gpu, new_oracle, cases.setup_cavity, 200, n=32, tol=1e-10)
     # the velocity as a vector field and the pressure within 1e-8; per
     # component within 1e-7 (w is about 1/32 of |u| here, and each step's
