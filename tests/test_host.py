@@ -45,3 +45,20 @@ def test_cavity_init_array_equals_init_func(I, cases, new_oracle):
     cases.setup_cavity(b, n=6, init="func")
     for name in ("U1", "U2", "U3", "P"):
         assert np.array_equal(a.get(name), b.get(name))
+
+
+def test_bench_step_roofline_arithmetic():
+    """The whole-step figure of the bench line on hand-made timer entries:
+    kernels a and c state bytes (4 launches of 1 GB, 2 of 0.5 GB = 5 GB over 2
+    steps in 4 ms of kernel time), b states none (1 ms); steps of 10 ms."""
+    import bench
+    stats = [{"name": "a", "ms": 2.0, "count": 4, "bytes": 1e9},
+             {"name": "b", "ms": 1.0, "count": 2, "bytes": 0.0},
+             {"name": "c", "ms": 2.0, "count": 2, "bytes": 0.5e9}]
+    r = bench.step_roofline(stats, 2, 1000, np.array([10.0, 10.0, 12.0]))
+    assert r["bytes_per_cell_step"] == 2.5e6
+    assert r["achieved_kernels"] == 1250.0 and r["frac_kernels"] == 0.1562
+    assert r["achieved_step"] == 250.0 and r["frac_step"] == 0.0312
+    assert r["kernel_time_with_bytes"] == 0.8 and r["peak"] == 8000.0
+    assert bench.step_roofline([], 2, 1000, np.array([10.0])) is None
+    assert bench.step_roofline(stats, 0, 1000, np.array([10.0])) is None
