@@ -18,8 +18,10 @@
  * Every case runs in IBM3D_PREC_MIXED_KRYLOV with BiCGSTAB + multigrid, tol
  * 1e-10, dt 0.005 (cavity) or 0.002 (sphere). One line per job is printed
  * when all have finished: id, kind, state, times, iterations, max|div U|,
- * device bytes and the error of a failed job. No counterpart in the
- * reference, whose drivers run one case per process.
+ * device bytes and the error of a failed job. A malformed or unknown case
+ * line ends the reading of CASEFILE with an error on stderr: the jobs read
+ * before it still run and are reported, and the exit status is 2. No
+ * counterpart in the reference, whose drivers run one case per process.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -71,6 +73,7 @@ int main(int argc, char **argv) {
     char kinds[MAX_JOBS][16];
     int njobs = 0;
     char kind[16];
+    int bad_input = 0;
     while (njobs < MAX_JOBS && fscanf(in, "%15s", kind) == 1) {
         IBMSolver *s = NULL;
         IBMServeInit init = {0, 0, 0, 0, 0, 0};
@@ -78,7 +81,11 @@ int main(int argc, char **argv) {
         double Re = 100;
         if (!strcmp(kind, "cavity") || !strcmp(kind, "badperiodic")) {
             const int bad = !strcmp(kind, "badperiodic");
-            if (fscanf(in, "%d %d", &n[0], &steps) != 2 || (!bad && fscanf(in, "%lf", &Re) != 1)) break;
+            if (fscanf(in, "%d %d", &n[0], &steps) != 2 || (!bad && fscanf(in, "%lf", &Re) != 1)) {
+                fprintf(stderr, "error: malformed %s case (expected %s)\n", kind, bad ? "N STEPS" : "N STEPS RE");
+                bad_input = 1;
+                break;
+            }
             n[1] = n[2] = n[0];
             const double box[6] = {0, 1, 0, 1, 0, 1};
             s = configure(n[0], n[1], n[2], box, Re, 0.005);
@@ -89,7 +96,11 @@ int main(int argc, char **argv) {
             init.amp = 1e-3;
             init.seed = 2020;
         } else if (!strcmp(kind, "sphere")) {
-            if (fscanf(in, "%d %d %d %d %lf", &n[0], &n[1], &n[2], &steps, &Re) != 5) break;
+            if (fscanf(in, "%d %d %d %d %lf", &n[0], &n[1], &n[2], &steps, &Re) != 5) {
+                fprintf(stderr, "error: malformed sphere case (expected NX NY NZ STEPS RE)\n");
+                bad_input = 1;
+                break;
+            }
             const double box[6] = {-5, 15, -5, 5, -5, 5};
             s = configure(n[0], n[1], n[2], box, Re, 0.002);
             IBMSolver_set_bc(s, DIR_WEST, BC_VELOCITY_COMPONENT, VAL_CONST, 1.0, 0.0, 0.0);
@@ -99,6 +110,7 @@ int main(int argc, char **argv) {
             init.u1 = 1.0;
         } else {
             fprintf(stderr, "error: unknown case kind '%s'\n", kind);
+            bad_input = 1;
             break;
         }
         const double bytes = IBMSolver_estimate_device_bytes(n[0], n[1], n[2], IBM3D_PREC_MIXED_KRYLOV, 0);
@@ -126,5 +138,5 @@ int main(int argc, char **argv) {
     }
     printf("jobs %d failed %d peak_admitted_bytes %.0f\n", njobs, failed, IBMServer_peak_bytes(srv));
     IBMServer_destroy(srv);
-    return 0;
+    return bad_input ? 2 : 0;
 }

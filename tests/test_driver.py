@@ -125,6 +125,21 @@ def test_serve_driver_failed_jobs_fail_alone(tmp_path):
     assert "jobs 2 failed 2" in r.stdout
 
 
+@pytest.mark.parametrize("line, message", [("cavity 16 5", "malformed cavity case"),
+                                           ("sphere 48 24 24 6", "malformed sphere case"),
+                                           ("tunnel 16 5", "unknown case kind 'tunnel'")])
+def test_serve_driver_malformed_input_is_an_error(tmp_path, line, message):
+    """A case line the C serving driver cannot read ends the reading with an
+    error on stderr and exit status 2; the job read before it still runs (here
+    a set-up that fails before any device work) and is reported."""
+    subprocess.run(["make", "-C", DRIVERS, "CC=gcc -Werror"], check=True, capture_output=True)
+    r, jobs = _serve(tmp_path, ["badperiodic 8 3", line, "badperiodic 10 2"])
+    assert r.returncode == 2
+    assert message in r.stderr
+    assert len(jobs) == 1 and next(iter(jobs.values()))["state"] == "failed"
+    assert "jobs 1 failed 1" in r.stdout
+
+
 @pytest.mark.gpu
 def test_serve_driver_runs_a_mix(tmp_path, I, cases):
     """The C serving driver on the GPU: two cavities and a sphere run beside a
