@@ -53,8 +53,13 @@ int main(int argc, char **argv) {
         if (!strcmp(argv[a], "-steps") && a + 1 < argc) steps_override = atoi(argv[++a]);
         else if (!strcmp(argv[a], "-precision") && a + 1 < argc) {
             const char *p = argv[++a];
-            precision = !strcmp(p, "mixed") ? IBM3D_PREC_MIXED
-                      : !strcmp(p, "mixed-krylov") ? IBM3D_PREC_MIXED_KRYLOV : IBM3D_PREC_FP64;
+            if (!strcmp(p, "fp64")) precision = IBM3D_PREC_FP64;
+            else if (!strcmp(p, "mixed")) precision = IBM3D_PREC_MIXED;
+            else if (!strcmp(p, "mixed-krylov")) precision = IBM3D_PREC_MIXED_KRYLOV;
+            else {
+                fprintf(stderr, "error: unknown precision '%s' (fp64, mixed or mixed-krylov)\n", p);
+                return 2;
+            }
         } else input = argv[a];
     }
 

@@ -55,6 +55,10 @@ def test_driver_builds_and_rejects_missing_input(tmp_path):
     assert os.access(EXE, os.X_OK)
     r = subprocess.run([EXE, str(tmp_path / "nope.in")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "cannot open" in r.stderr
+    # an unknown precision is refused before anything is read
+    r = subprocess.run([EXE, str(tmp_path / "nope.in"), "-precision", "fp16"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "unknown precision 'fp16'" in r.stderr
 
 
 def test_sample_input_parses():
