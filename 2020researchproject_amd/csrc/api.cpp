@@ -231,6 +231,7 @@ void IBMSolver_set_mg_params(IBMSolver *s, double gamma, int pre, int post, int 
     if (s->assembled) fatal("IBMSolver_set_mg_params must precede IBMSolver_assemble");
     if (pre < 0 || post < 0 || pre + post == 0 || pre > 16 || post > 16) fatal("invalid multigrid sweep counts");
     if (smoother != 0 && smoother != 1) fatal("multigrid smoother must be 0 or 1");
+    if (!std::isfinite(gamma)) fatal("multigrid coarsening threshold gamma must be finite");
     s->mg_gamma = gamma;
     s->mg_pre = pre;
     s->mg_post = post;
@@ -250,6 +251,7 @@ int IBMSolver_get_pressure_mg_bits(IBMSolver *s) {
 
 void IBMSolver_set_mg_light_levels(IBMSolver *s, int from_level) {
     if (s->assembled) fatal("IBMSolver_set_mg_light_levels must precede IBMSolver_assemble");
+    if (from_level < 0) fatal("multigrid light levels: from_level must be 0 or positive");
     s->mg_light_from = from_level;
 }
 

@@ -56,14 +56,15 @@ void IBMSolver_set_precision(IBMSolver *, int mode);
    red-black Gauss-Seidel sweeps before / after the coarse correction;
    smoother 0: each sweep is one fused launch (both colours), ghost rows
    relaxed after each sweep; 1: one launch per colour, ghost rows relaxed
-   after each colour. Defaults 3.5, 2, 2, 1. Must be called before
-   IBMSolver_assemble. */
+   after each colour. Defaults 3.5, 2, 2, 1. gamma must be finite; pre and
+   post 0..16, not both 0. Must be called before IBMSolver_assemble. */
 void IBMSolver_set_mg_params(IBMSolver *, double gamma, int pre, int post, int smoother);
 /* Levels from `from_level` down (when from_level > 0) smooth with one sweep
    before and one after the coarse correction instead of pre/post; 0 smooths
    every level with pre/post. Default 1: the finest level takes pre/post
    sweeps, every coarser level one (cheaper cycles at the same iteration
-   count on the sphere and cavity cases). Before IBMSolver_assemble. */
+   count on the sphere and cavity cases); from_level >= 0. Before
+   IBMSolver_assemble. */
 void IBMSolver_set_mg_light_levels(IBMSolver *, int from_level);
 /* Over-relaxation factor omega of the red-black smoother of the pressure
    multigrid (smoother 1): a fluid row takes x + omega (x_GS - x) instead of

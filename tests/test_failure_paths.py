@@ -89,6 +89,26 @@ def test_odd_periodic_axis_with_multigrid_is_refused():
     assert "periodic x axis with an odd cell count (9)" in r.stderr
 
 
+@pytest.mark.parametrize("call, message", [
+    ("s.set_mg_params(float('nan'), 2, 2, 1)", "gamma must be finite"),
+    ("s.set_mg_params(float('inf'), 2, 2, 1)", "gamma must be finite"),
+    ("s.set_mg_params(3.5, 0, 0, 1)", "invalid multigrid sweep counts"),
+    ("s.set_mg_params(3.5, 2, 2, 2)", "multigrid smoother must be 0 or 1"),
+    ("s.set_mg_light_levels(-1)", "from_level must be 0 or positive"),
+], ids=["gamma-nan", "gamma-inf", "no-sweeps", "smoother-2", "light-negative"])
+def test_multigrid_parameters_out_of_range_are_refused(call, message):
+    """The multigrid setters print and abort on arguments outside their
+    documented ranges (a non-finite coarsening threshold would stop the
+    coarsening and leave a dense coarse solve of the fine grid's size)."""
+    r = run_case(HIP_LIB, f"""
+        {call}
+        print("accepted")
+    """)
+    assert aborted(r), (r.returncode, r.stdout, r.stderr)
+    assert message in r.stderr
+    assert "accepted" not in r.stdout
+
+
 @pytest.mark.parametrize("omega", [0.0, 2.0, -0.5, float("nan")])
 def test_multigrid_over_relaxation_outside_0_2_is_refused(omega):
     """IBMSolver_set_mg_relaxation accepts 0 < omega < 2 (the range where
