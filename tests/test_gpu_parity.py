@@ -527,15 +527,17 @@ def test_concurrent_momentum_solves_match_one_at_a_time(new_gpu, cases, precisio
         assert np.array_equal(a, b, equal_nan=True)
 
 
+@pytest.mark.parametrize("precision", [0, 2], ids=["fp64", "fp64-krylov-fp32-mg"])
 @pytest.mark.parametrize("smoother", [0, 1], ids=["fused-sweep", "per-colour"])
-def test_multigrid_smoothers_reach_the_same_solution(new_gpu, cases, smoother):
+def test_multigrid_smoothers_reach_the_same_solution(new_gpu, cases, smoother, precision):
     """Both red-black smoothers (one fused launch per sweep, or one launch
     per colour) are preconditioners only: the converged fields agree with
-    the per-colour default to well below the solver tolerance."""
+    the per-colour default to well below the solver tolerance, in FP64 and
+    in the FP32 cycles (and FP32 momentum sweeps) of the mixed path."""
     ref, g = new_gpu(), new_gpu()
-    cases.setup_sphere(ref, 64, 32, 32, tol=1e-12)
+    cases.setup_sphere(ref, 64, 32, 32, tol=1e-12, precision=precision)
     g.set_mg_params(3.0, 2, 2, smoother)
-    cases.setup_sphere(g, 64, 32, 32, tol=1e-12)
+    cases.setup_sphere(g, 64, 32, 32, tol=1e-12, precision=precision)
     ref.iterate(3)
     g.iterate(3)
     compare(g, ref, 1e-9, names=("U1", "U2", "U3", "P"))
