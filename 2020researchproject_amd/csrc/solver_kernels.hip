@@ -79,9 +79,6 @@ static inline dim3 rgrid(const Dims &d) { return dim3((d.nx + 63) / 64, (d.ny + 
 // lane; measured 25 us against 29 us per FP64 launch at 256 x 128 x 128
 // with four planes, whose registers halve the occupancy).
 constexpr int kRedZP = 2;
-static inline dim3 pgrid(const Dims &d) {
-    return dim3(((d.nx + 1) / 2 + 63) / 64, (d.ny + 3) / 4, (d.nz + kRedZP - 1) / kRedZP);
-}
 static inline long pblocks(const Dims &d) {
     return (long)(((d.nx + 1) / 2 + 63) / 64) * ((d.ny + 3) / 4) * ((d.nz + kRedZP - 1) / kRedZP);
 }
@@ -197,18 +194,14 @@ __device__ inline T stencil_diag(const SysCoef<T> &S, int i, int j, int k, long 
 // i + j + k even), the value the one-colour smoother writes at (i, j, k):
 // same arithmetic (the neighbour sum is 0), so a kernel that produces a
 // level's right-hand side can also write this colour and save that launch.
-// first_colour_of: the same from a diagonal formed beforehand (its loads can
-// then issue ahead of those that produce bv).
+// The diagonal (stencil_diag) is formed beforehand, so its loads can issue
+// ahead of those that produce bv.
 template <typename T>
 __device__ inline T first_colour_of(const SysCoef<T> &S, unsigned f, int i, int j, int k, long q, T bv, T diag) {
     if ((i + j + k) & 1) return T(0);
     if (q == S.pin || f == F_SOLID) return bv;
     if (f != F_FLUID) return T(0);
     return sor_value(S.omega, T(0), diag != T(0) ? (bv + T(0)) / diag : T(0));
-}
-template <typename T>
-__device__ inline T first_colour(const SysCoef<T> &S, unsigned f, int i, int j, int k, long q, T bv) {
-    return first_colour_of(S, f, i, j, k, q, bv, stencil_diag(S, i, j, k, q));
 }
 
 // ---------------------------------------------------------------------------
